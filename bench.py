@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark: MPix/s of the PolyMage unsharp-mask pipeline on one MI355X.
+
+Workload: unsharp mask on an RGB float32 3x2848x4256 smooth random field
+(threshold 0.001, weight 3), the reference application's expressions, one
+fused group, 32x256 tiles (or the tuned schedule with --schedule).  A step
+is one pass of the pipeline over the whole image with input and output
+resident in HBM; buffer sets are rotated so each step reads cold data.
+
+Prints one JSON line.  ``roofline`` uses the ideal traffic (input bytes +
+output bytes) over the kernel time measured with HIP events on the stream
+the kernel runs on; ``cpu_baseline`` times the oracle's C++/OpenMP port of
+the same pipeline on the host cores over a bounded sample.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--workload', default='unsharp', choices=['unsharp', 'harris'])
+    ap.add_argument('--tiles', default=None, help='e.g. 32x256')
+    ap.add_argument('--threads', type=int, default=None)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--traffic', default=None,
+                    help='JSON file with measured FETCH/WRITE KiB per dispatch')
+    return ap.parse_args()
+
+
+def cpu_baseline(w, budget_s):
+    """Oracle port (C++/OpenMP, reference compile flags) on host cores."""
+    import numpy as np
+    from oracle import cemit
+    threads = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault('OMP_NUM_THREADS', str(threads))
+    os.environ.setdefault('OMP_PROC_BIND', 'close')
+    prog = cemit.build([w.out_func], 'bench_cpu_%s' % w.name, flags=cemit.REFERENCE_FLAGS)
+    vals = dict(w.values)
+    vals[w.out_func.name] = np.zeros(w.out_shape, w.out_func.typ.np_dtype())
+    prog(vals)     # warm-up
+    best = None
+    runs = 0
+    t_start = time.perf_counter()
+    while runs < 10 and time.perf_counter() - t_start < budget_s:
+        t0 = time.perf_counter()
+        prog(vals)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+        runs += 1
+    return {'value': round(w.mpix / best, 3), 'unit': 'MPix/s', 'cores': int(os.environ['OMP_NUM_THREADS']),
+            'kind': 'port',
+            'sample': '%d full-size runs of the oracle C++/OpenMP port of the same pipeline '
+                      '(g++ -O3 -fopenmp -march=native), min time' % runs}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU only; --gpus %d is not supported '
+              '(multi-GPU sharding is described in DESIGN.md)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from polymage_amd import runtime
+    from polymage_amd.apps import workloads
+
+    gopts = {}
+    if args.threads:
+        gopts['threads'] = args.threads
+    tiles = tuple(int(t) for t in args.tiles.split('x')) if args.tiles else \
+        workloads.DEFAULT_TILES[args.workload]
+    w = getattr(workloads, args.workload)(tiles=tiles, gpu_options=gopts or None)
+    bp = runtime.build(w.pipe)
+
+    dev = torch.device('cuda:0')
+    torch.cuda.set_device(dev)
+    inputs = [f for f in bp.inputs]
+    out = bp.outputs[0]
+    in_bytes = sum(w.values[f.name].nbytes for f in inputs)
+    out_bytes = int(np.prod(w.out_shape)) * out.typ.nbytes()
+    # rotate buffer sets so the 256 MB Infinity Cache holds no step's data
+    nsets = max(2, min(10, int((1 << 30) // (in_bytes + out_bytes)) + 1))
+    sets = []
+    for _ in range(nsets):
+        d = {}
+        for f in inputs:
+            d[f.name] = torch.from_numpy(w.values[f.name]).to(dev)
+        d[out.name] = torch.empty(w.out_shape, dtype=getattr(torch, np.dtype(out.typ.np_dtype()).name), device=dev)
+        sets.append(d)
+    scal = dict((k, v) for k, v in w.values.items() if not hasattr(v, 'shape'))
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(i):
+        vals = dict(scal)
+        vals.update(sets[i % nsets])
+        bp.run_dev(vals, stream)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for i in range(args.steps):
+        step(i)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / args.steps
+    value = w.mpix / (ms_per_step / 1e3)
+
+    # per-launch kernel time (single kernel per step for this workload)
+    nk = len(bp.code.kernels)
+    achieved = w.ideal_bytes / (ms_per_step / 1e3) / 1e9
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get(w.name)
+
+    result = {
+        'metric': 'MPix/s per pipeline, 1 MI355X: learned vs exhaustive schedule; % HBM roofline',
+        'value': round(value, 3),
+        'unit': 'MPix/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms_per_step, 5),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded smooth random field, rotated over %d HBM-resident buffer sets)' % nsets,
+        'config': {'workload': '%s %s' % (w.name, 'x'.join(str(s) for s in w.out_shape)),
+                   'tiles': list(tiles), 'kernels': nk, 'schedule': 'default',
+                   'wall_s': round(wall, 4)},
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+                     'traffic': traffic},
+    }
+    if not args.no_cpu:
+        try:
+            result['cpu_baseline'] = cpu_baseline(w, args.cpu_seconds)
+        except Exception as e:  # the CPU leg must not hide the GPU result
+            result['cpu_baseline'] = {'value': None, 'error': str(e)[:200]}
+    print(json.dumps(result))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,76 @@
+"""The benchmark workloads at their configured sizes.
+
+Each entry builds the DSL pipeline with the schedule named in the config
+and describes the data: arrays, MPix (output spatial pixels) and the
+ideal HBM traffic (all input bytes + bytes of the output region written).
+"""
+
+import numpy as np
+
+from .. import buildPipeline
+from ..constructs import Condition
+from . import inputs
+
+
+class Workload(object):
+    def __init__(self, name, pipe, out_func, values, out_shape, mpix, ideal_bytes,
+                 defined=None, dtype='f32'):
+        self.name = name
+        self.pipe = pipe
+        self.out_func = out_func
+        self.values = values            # host arrays + scalar params
+        self.out_shape = out_shape
+        self.mpix = mpix
+        self.ideal_bytes = ideal_bytes
+        self.defined = defined          # slice tuple of the written region
+        self.dtype = dtype
+
+
+def _pipe(out, pd, sizes, name, tiles, group_size, literal_mode='double', gpu_options=None,
+          extra_est=()):
+    est = [(pd[k], v) for k, v in sizes.items()] + list(extra_est)
+    cons = [Condition(pd[k], '==', v) for k, v in sizes.items()]
+    return buildPipeline([out], param_estimates=est, param_constraints=cons,
+                         tile_sizes=tiles, group_size=group_size, pipe_name=name,
+                         literal_mode=literal_mode, gpu_options=gpu_options)
+
+
+def harris(rows=2048, cols=2048, tiles=(32, 256), gpu_options=None, with_data=True):
+    from .harris import harris_pipe
+    pd = {}
+    out = harris_pipe(pd)
+    R, C = rows - 2, cols - 2
+    pipe = _pipe(out, pd, {'R': R, 'C': C}, 'harris', list(tiles), 11, gpu_options=gpu_options)
+    vals = {'R': R, 'C': C}
+    if with_data:
+        vals['img'] = inputs.harris_input(rows, cols)
+    out_px = (R - 2) * (C - 2)
+    ideal = rows * cols * 4 + out_px * 4
+    return Workload('harris', pipe, out, vals, (rows, cols), out_px / 1e6, ideal,
+                    defined=(slice(2, R), slice(2, C)))
+
+
+def unsharp(rows=2848, cols=4256, tiles=(32, 256), gpu_options=None, with_data=True,
+            group_size=5):
+    from .unsharp import unsharp_mask_full
+    pd = {}
+    out = unsharp_mask_full(pd)
+    R, C = rows - 4, cols - 4
+    pipe = _pipe(out, pd, {'R': R, 'C': C}, 'unsharp', list(tiles), group_size,
+                 gpu_options=gpu_options)
+    vals = {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0}
+    if with_data:
+        vals['img'] = inputs.unsharp_input(rows, cols)
+    out_px = R * C
+    ideal = 3 * rows * cols * 4 + 3 * R * C * 4
+    return Workload('unsharp', pipe, out, vals, (3, R, C), out_px / 1e6, ideal)
+
+
+DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256)}
+
+
+def prebuild_default():
+    from .. import runtime
+    for fn in (harris, unsharp):
+        w = fn(with_data=False)
+        runtime.build(w.pipe)

@@ -1,0 +1,145 @@
+"""DSL expression -> C/HIP expression text with the reference's semantics.
+
+The reference emits every DSL operator verbatim as a C operator
+(codegen.py:574-581), so C's own rules (integer promotion, truncating
+``/`` and ``%``, float*double -> double) decide the arithmetic.  We keep
+that, and keep the three places where the reference adds its own rules:
+
+  * literals: Python ``str`` with no suffix, i.e. C doubles
+    (targetc.py:80-86); 'float12' mode prints ``%.12g`` + ``f`` like the
+    reference's checked-in generated files;
+  * ``Select`` (and Max/Min, which are Selects) computes both branches into
+    temporaries whose C type is the branch's DSL type before selecting
+    (codegen.py:618-642); inside ``Powf`` arguments no temporaries are made
+    (codegen.py:654-657);
+  * ``Cast(T, e)`` -> ``(T) (e)``.
+
+Math builtins are printed through small device helpers that pin the C++
+overload the reference's g++ build resolves to (``log``/``pow`` always in
+double, ``std::exp``/``std::sqrt``/``std::abs`` by argument type).
+"""
+
+from .expr_ast import (AbstractBinaryOpNode, AbstractUnaryOpNode, Value,
+                       Variable, Parameter, Reference, Condition, Select, Cast,
+                       InbuiltFunction, Pow, Powf, Log, Exp, Sin, Cos, Sqrt,
+                       Sqrtf, Abs, RandomFloat)
+from .expr_types import Float
+from .expression import getType
+
+
+def c_type(t):
+    return t.c_name()
+
+
+def literal(v, mode):
+    if isinstance(v.value, float):
+        if mode == 'float12' and v.typ is Float:
+            s = '%.12g' % v.value
+            if 'e' not in s and '.' not in s:
+                s += '.0'
+            return s + 'f'
+        s = repr(v.value)
+        if s in ('inf', '-inf', 'nan'):
+            raise ValueError("non-finite literal")
+        return s
+    return str(v.value)
+
+
+class Printer(object):
+    """Prints expressions; ``ref_cb(ref, printer, prologue)`` returns the C
+    text of a Reference, ``var_names`` maps Variables to C names."""
+
+    def __init__(self, var_names, ref_cb, mode='double', param_prefix=''):
+        self.var_names = var_names
+        self.ref_cb = ref_cb
+        self.mode = mode
+        self.param_prefix = param_prefix
+        self._tmp = [0]
+
+    def tmp(self):
+        self._tmp[0] += 1
+        return '_t%d' % self._tmp[0]
+
+    def expr(self, e, pro=None):
+        e = Value.numericToValue(e)
+        if isinstance(e, Value):
+            return literal(e, self.mode)
+        if isinstance(e, Parameter):
+            return self.param_prefix + e.name
+        if isinstance(e, Variable):
+            return self.var_names[e]
+        if isinstance(e, AbstractBinaryOpNode):
+            return '(' + self.expr(e.left, pro) + ' ' + e.op + ' ' + \
+                self.expr(e.right, pro) + ')'
+        if isinstance(e, AbstractUnaryOpNode):
+            return e.op + '(' + self.expr(e.child, pro) + ')'
+        if isinstance(e, Reference):
+            return self.ref_cb(e, self, pro)
+        if isinstance(e, Select):
+            c = self.cond(e.condition, pro)
+            t = self.expr(e.true_expression, pro)
+            f = self.expr(e.false_expression, pro)
+            if pro is None:
+                return '(' + c + '? ' + t + ': ' + f + ')'
+            a, b, s = self.tmp(), self.tmp(), self.tmp()
+            pro.append('%s %s = %s;' % (c_type(getType(e.true_expression)), a, t))
+            pro.append('%s %s = %s;' % (c_type(getType(e.false_expression)), b, f))
+            pro.append('%s %s = (%s? %s: %s);' % (c_type(getType(e)), s, c, a, b))
+            return s
+        if isinstance(e, Cast):
+            return '(' + c_type(e.typ) + ') (' + self.expr(e.expression, pro) + ')'
+        if isinstance(e, Powf):
+            return 'powf(' + self.expr(e.arguments[0], None) + ', ' + \
+                self.expr(e.arguments[1], None) + ')'
+        if isinstance(e, Pow):
+            return 'pow((double)(' + self.expr(e.arguments[0], pro) + '), (double)(' + \
+                self.expr(e.arguments[1], pro) + '))'
+        if isinstance(e, Log):
+            return 'log((double)(' + self.expr(e.arguments[0], pro) + '))'
+        if isinstance(e, Exp):
+            return 'pmx_exp(' + self.expr(e.arguments[0], pro) + ')'
+        if isinstance(e, Sin):
+            return 'pmx_sin(' + self.expr(e.arguments[0], pro) + ')'
+        if isinstance(e, Cos):
+            return 'pmx_cos(' + self.expr(e.arguments[0], pro) + ')'
+        if isinstance(e, Sqrt):
+            return 'pmx_sqrt(' + self.expr(e.arguments[0], pro) + ')'
+        if isinstance(e, Sqrtf):
+            return 'sqrtf(' + self.expr(e.arguments[0], pro) + ')'
+        if isinstance(e, Abs):
+            return 'pmx_abs(' + self.expr(e.arguments[0], pro) + ')'
+        if isinstance(e, RandomFloat):
+            raise NotImplementedError("RandomFloat has no GPU lowering")
+        raise TypeError(type(e))
+
+    def cond(self, c, pro=None):
+        if c.conditional in ('&&', '||'):
+            return '(' + self.cond(c.lhs, pro) + ' ' + c.conditional + ' ' + \
+                self.cond(c.rhs, pro) + ')'
+        return '(' + self.expr(c.lhs, pro) + ' ' + c.conditional + ' ' + \
+            self.expr(c.rhs, pro) + ')'
+
+
+# Device helpers that pin the overloads libstdc++ picks on the host.
+DEVICE_PRELUDE = r'''
+#define PMX_BIG 536870912
+__device__ __forceinline__ int pmx_abs(int v) { return v < 0 ? -v : v; }
+__device__ __forceinline__ long long pmx_abs(long long v) { return v < 0 ? -v : v; }
+__device__ __forceinline__ float pmx_abs(float v) { return __builtin_fabsf(v); }
+__device__ __forceinline__ double pmx_abs(double v) { return __builtin_fabs(v); }
+__device__ __forceinline__ float pmx_exp(float v) { return expf(v); }
+__device__ __forceinline__ double pmx_exp(double v) { return exp(v); }
+__device__ __forceinline__ double pmx_exp(int v) { return exp((double)v); }
+__device__ __forceinline__ float pmx_sqrt(float v) { return sqrtf(v); }
+__device__ __forceinline__ double pmx_sqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ double pmx_sqrt(int v) { return sqrt((double)v); }
+__device__ __forceinline__ float pmx_sin(float v) { return sinf(v); }
+__device__ __forceinline__ double pmx_sin(double v) { return sin(v); }
+__device__ __forceinline__ double pmx_sin(int v) { return sin((double)v); }
+__device__ __forceinline__ float pmx_cos(float v) { return cosf(v); }
+__device__ __forceinline__ double pmx_cos(double v) { return cos(v); }
+__device__ __forceinline__ double pmx_cos(int v) { return cos((double)v); }
+__device__ __forceinline__ int pmx_imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int pmx_imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int pmx_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+'''

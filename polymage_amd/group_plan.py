@@ -1,0 +1,450 @@
+"""Planning of one fused stencil group for the GPU: tile, regions, storage.
+
+One group becomes one HIP kernel and one workgroup per overlapped tile
+(the GPU form of the reference's OpenMP tile loop, codegen.py:411-551 and
+poly_schedule.py:429-536).  Planning steps:
+
+1. anchor: the group's last live-out; the tile is a box of the anchor's
+   domain.  ``tile_sizes`` are consumed by anchor dimensions longer than
+   the tile, in order, exactly as overlap_tile does (poly_schedule.py:
+   433-451); untiled dimensions along which every in-group dependence is
+   the identity (channels) are split one point per workgroup.
+2. alignment/scale: every stage dimension that is a monotone image of an
+   anchor dimension inherits its alignment and rational scale (the role of
+   align_scale.py:93-854).  Live-outs other than the anchor own the points
+   whose scaled anchor coordinate falls in the tile, so tiles partition
+   every live-out and each point is written once.
+3. regions: exact per-tile boxes back-propagated through the references of
+   each Case part (``region_ir``), clipped to the domain and to the hull of
+   the stage's Case boxes.  No stage reads an uncomputed producer point,
+   unlike the reference trapezoid (its edge points read scratch index -1).
+4. inlining: stages read only through identity accesses and not live-out
+   are evaluated in registers inside their consumers (cast to their own
+   type, so rounding is unchanged); the others are materialised in LDS.
+5. phases: materialised stages are ordered in phases separated by
+   workgroup barriers; LDS buffers are reused across phases by storage
+   class and liveness, like storage_mapping.py:209-682 reuses scratchpads.
+"""
+
+from fractions import Fraction
+import itertools
+import math
+
+from .analysis import (PlanError, classify_access, case_parts, ceval_int,
+                       eval_bexpr)
+from .constructs import Reference, Reduction
+from .region_ir import RegionIR, BIG
+
+LDS_LIMIT_BYTES = 160 * 1024
+
+
+class StageInfo(object):
+    def __init__(self, comp, index):
+        self.comp = comp
+        self.func = comp.func
+        self.index = index
+        self.nd = len(comp.func.variables)
+        self.parts = comp.parts
+        self.liveout = False
+        self.inline = False
+        self.consumers = []        # [(StageInfo consumer, part, Reference)]
+        self.align = [None] * self.nd   # (anchor dim, Fraction scale)
+        self.region = None              # [(lo_id, hi_id)]
+        self.own = None
+        self.static_ext = None
+        self.phase = None
+        self.lds_offset = None
+        self.lds_class = None
+        self.needs_lds = False
+
+    @property
+    def name(self):
+        return self.func.name
+
+    def __repr__(self):
+        return 'Stage(%s)' % self.name
+
+
+class GroupPlan(object):
+    def __init__(self, pipe, group, gindex, tile_sizes=None, opts=None):
+        self.pipe = pipe
+        self.group = group
+        self.gindex = gindex
+        self.opts = dict(pipe.gpu_options)
+        if opts:
+            self.opts.update(opts)
+        self.tile_sizes = list(tile_sizes if tile_sizes is not None else pipe.tile_sizes)
+        self.env = pipe.param_env
+        comps = group.comps
+        if any(c.is_reduction for c in comps) or any(c.is_const for c in comps):
+            raise PlanError("not a stencil group")
+        self.stages = [StageInfo(c, i) for i, c in enumerate(comps)]
+        self.by_comp = dict((s.comp, s) for s in self.stages)
+        liveouts = pipe.liveouts(group)
+        for s in self.stages:
+            s.liveout = s.comp in liveouts
+        self._consumers()
+        self._anchor()
+        self._align()
+        self._grid()
+        self._inline()
+        self._regions()
+        self._sizes()
+        self._phases()
+        self._storage()
+
+    # -- structure -------------------------------------------------------------
+    def in_group(self, func):
+        c = self.pipe.func_map.get(func)
+        return c is not None and c in self.by_comp
+
+    def stage_of(self, func):
+        return self.by_comp[self.pipe.func_map[func]]
+
+    def _part_refs(self, part):
+        refs = part.expr.collect(Reference)
+        for c in part.pred:
+            refs += [r for r in c.collect(Reference) if r not in refs]
+        return refs
+
+    def _consumers(self):
+        for s in self.stages:
+            for part in s.parts:
+                for r in self._part_refs(part):
+                    if self.in_group(r.objectRef):
+                        p = self.stage_of(r.objectRef)
+                        p.consumers.append((s, part, r))
+
+    def _anchor(self):
+        lo = [s for s in self.stages if s.liveout]
+        self.anchor = max(lo, key=lambda s: (s.comp.level, self.pipe.comp_size(s.comp), -s.index))
+
+    def _grid(self):
+        best = self.anchor
+        dom = self.pipe.domain_box(best.func)
+        self.grid_dims = []      # [(anchor dim, tile size)]
+        k = 0
+        for d in range(best.nd):
+            ext = dom[d][1] - dom[d][0] + 1
+            if k < len(self.tile_sizes) and ext > self.tile_sizes[k]:
+                self.grid_dims.append((d, int(self.tile_sizes[k])))
+                k += 1
+            elif self._identity_dim(d):
+                self.grid_dims.append((d, 1))
+
+    def _identity_dim(self, d):
+        """True when every in-group dependence along anchor dim d is the
+        identity, so splitting it one point per tile costs no halo."""
+        for s in self.stages:
+            for (c, part, r) in s.consumers:
+                for i, arg in enumerate(r.arguments):
+                    if s.align[i] is None or s.align[i][0] != d:
+                        continue
+                    acc = classify_access(arg, c.func.variables)
+                    if acc.kind != 'var' or not acc.identity:
+                        return False
+                    if c.align[acc.dim] is None or c.align[acc.dim][0] != d:
+                        return False
+        return True
+
+    def _align(self):
+        a = self.anchor
+        a.align = [(d, Fraction(1)) for d in range(a.nd)]
+        for s in reversed(self.stages):
+            if s is a:
+                continue
+            cand = [None] * s.nd
+            for (c, part, r) in s.consumers:
+                for i, arg in enumerate(r.arguments):
+                    acc = classify_access(arg, c.func.variables)
+                    if acc.kind != 'var' or c.align[acc.dim] is None:
+                        continue
+                    ad, sc = c.align[acc.dim]
+                    if cand[i] is None:
+                        cand[i] = (ad, sc * acc.slope)
+            s.align = cand
+
+    def _inline(self):
+        if self.opts.get('no_inline'):
+            return
+        for s in self.stages:
+            if s.liveout or not s.consumers:
+                continue
+            ok = True
+            for (c, part, r) in s.consumers:
+                if len(r.arguments) != c.nd or s.nd != c.nd:
+                    ok = False
+                    break
+                for d, arg in enumerate(r.arguments):
+                    if arg is not c.func.variables[d]:
+                        ok = False
+                        break
+            s.inline = ok
+
+    # -- regions -----------------------------------------------------------------
+    def _regions(self):
+        ir = RegionIR()
+        self.ir = ir
+        pipe = self.pipe
+        a = self.anchor
+        big = ir.const(BIG)
+        nbig = ir.const(-BIG)
+        # anchor tile box and tile counts
+        self.ntiles = {}
+        tile_box = []
+        for d in range(a.nd):
+            lb = ir.expr(a.func.domain[d].lowerBound)
+            ub = ir.expr(a.func.domain[d].upperBound)
+            tile_box.append((lb, ub, None))
+        for (d, T) in self.grid_dims:
+            lb, ub, _ = tile_box[d]
+            t = ir.tile(d)
+            lo = ir.add(lb, ir.mul(t, T))
+            hi = ir.add(lo, ir.const(T - 1))
+            tile_box[d] = (lo, hi, t)
+        self.tile_box = tile_box
+
+        def dom_ids(s, i):
+            return (ir.expr(s.func.domain[i].lowerBound),
+                    ir.expr(s.func.domain[i].upperBound))
+
+        def case_hull(s, i):
+            lo_h = hi_h = None
+            dlo, dhi = dom_ids(s, i)
+            for part in s.parts:
+                blo, bhi = part.box[i]
+                plo = ir.expr(blo) if blo is not None else dlo
+                phi = ir.expr(bhi) if bhi is not None else dhi
+                lo_h = plo if lo_h is None else ir.min(lo_h, plo)
+                hi_h = phi if hi_h is None else ir.max(hi_h, phi)
+            return lo_h, hi_h
+
+        # ownership of live-outs (scaled tile ranges, open at the edges)
+        ntile_ids = {}
+        for (d, T) in self.grid_dims:
+            lb = ir.expr(a.func.domain[d].lowerBound)
+            ub = ir.expr(a.func.domain[d].upperBound)
+            ext = ir.add(ir.sub(ub, lb), ir.const(1))
+            nt = ir.fdiv(ir.add(ext, ir.const(T - 1)), T)
+            ntile_ids[d] = nt
+        self.ntile_ids = ntile_ids
+
+        def own_range(s, i):
+            al = s.align[i]
+            dlo, dhi = dom_ids(s, i)
+            if al is None:
+                return dlo, dhi
+            ad, sc = al
+            lo_t, hi_t, t = tile_box[ad]
+            if t is None:
+                return dlo, dhi
+            last = ir.sub(ntile_ids[ad], ir.const(1))
+            lo_open = ir.sel_eq(t, ir.const(0), nbig, lo_t)
+            hi_open = ir.sel_eq(t, last, big, hi_t)
+            if sc == 1:
+                return lo_open, hi_open
+            if sc <= 0:
+                raise PlanError("live-out %s has a non-positive scale" % s.name)
+            P, Q = sc.numerator, sc.denominator
+            # lo = ceil(lo_t * P / Q); hi = ceil((hi_t + 1) * P / Q) - 1
+            lo_s = ir.sub(ir.const(0), ir.fdiv(ir.sub(ir.const(0), ir.mul(lo_t, P)), Q))
+            hi_s = ir.sub(ir.sub(ir.const(0), ir.fdiv(ir.sub(ir.const(0),
+                                                            ir.mul(ir.add(hi_t, ir.const(1)), P)), Q)),
+                          ir.const(1))
+            lo_s = ir.sel_eq(t, ir.const(0), nbig, lo_s)
+            hi_s = ir.sel_eq(t, last, big, hi_s)
+            return lo_s, hi_s
+
+        for s in reversed(self.stages):
+            lo = [big] * s.nd
+            hi = [nbig] * s.nd
+            if s.liveout:
+                own = []
+                for i in range(s.nd):
+                    if s is a:
+                        lo_t, hi_t, t = tile_box[i]
+                        own.append((lo_t, hi_t))
+                    else:
+                        own.append(own_range(s, i))
+                s.own = own
+                for i in range(s.nd):
+                    lo[i] = ir.min(lo[i], own[i][0])
+                    hi[i] = ir.max(hi[i], own[i][1])
+            for (c, part, r) in s.consumers:
+                # sub-region of the consumer that this part covers
+                sub = []
+                for j in range(c.nd):
+                    clo, chi = c.region[j]
+                    blo, bhi = part.box[j]
+                    if blo is not None:
+                        clo = ir.max(clo, ir.expr(blo))
+                    if bhi is not None:
+                        chi = ir.min(chi, ir.expr(bhi))
+                    sub.append((clo, chi))
+                valid = [(l, h) for (l, h) in sub]
+                for i, arg in enumerate(r.arguments):
+                    acc = classify_access(arg, c.func.variables)
+                    if acc.kind == 'var':
+                        l, h = sub[acc.dim]
+                        al = ir.access(arg, acc.var, l)
+                        ah = ir.access(arg, acc.var, h)
+                        if not acc.increasing:
+                            al, ah = ah, al
+                    elif acc.kind == 'const':
+                        al = ah = ir.expr(arg)
+                    else:
+                        al, ah = dom_ids(s, i)
+                    lo[i] = ir.sel_valid(valid, ir.min(lo[i], al), lo[i])
+                    hi[i] = ir.sel_valid(valid, ir.max(hi[i], ah), hi[i])
+            reg = []
+            for i in range(s.nd):
+                dlo, dhi = dom_ids(s, i)
+                hlo, hhi = case_hull(s, i)
+                reg.append((ir.max(ir.max(lo[i], dlo), hlo),
+                            ir.min(ir.min(hi[i], dhi), hhi)))
+            s.region = reg
+
+    def tile_space(self, env=None):
+        env = self.env if env is None else env
+        vals = self.ir.evaluate(env, {})  if False else None
+        counts = []
+        a = self.anchor
+        for (d, T) in self.grid_dims:
+            lb = ceval_int(a.func.domain[d].lowerBound, env)
+            ub = ceval_int(a.func.domain[d].upperBound, env)
+            counts.append(max(0, (ub - lb + 1 + T - 1) // T))
+        return counts
+
+    def _sizes(self):
+        counts = self.tile_space()
+        self.tile_counts = counts
+        total = 1
+        for n in counts:
+            total *= n
+        self.total_tiles = total
+        # sample tiles: all if few, else a window around the middle and the
+        # edges of every grid dimension (region extents are periodic in the
+        # tile index with a period dividing the scale denominators)
+        per_dim = []
+        for n in counts:
+            if n <= 96:
+                per_dim.append(list(range(n)))
+            else:
+                mid = n // 2
+                s = set(range(0, 8)) | set(range(n - 8, n)) | set(range(mid - 40, mid + 40))
+                per_dim.append(sorted(x for x in s if 0 <= x < n))
+        ext = dict((s, [0] * s.nd) for s in self.stages)
+        nsamp = 1
+        for p in per_dim:
+            nsamp *= max(1, len(p))
+        if nsamp > 20000:
+            # thin the sample uniformly
+            step = int(math.ceil((nsamp / 20000.0) ** (1.0 / max(1, len(per_dim)))))
+            per_dim = [p[::step] + ([p[-1]] if p and p[-1] not in p[::step] else []) for p in per_dim]
+        dims = [d for (d, T) in self.grid_dims]
+        for combo in itertools.product(*per_dim):
+            tiles = dict(zip(dims, combo))
+            vals = self.ir.evaluate(self.env, tiles)
+            for s in self.stages:
+                for i, (l, h) in enumerate(s.region):
+                    e = vals[h] - vals[l] + 1
+                    if e > ext[s][i]:
+                        ext[s][i] = e
+        for s in self.stages:
+            s.static_ext = [max(1, e) for e in ext[s]]
+
+    # -- phases and storage ---------------------------------------------------------
+    def _phases(self):
+        memo = {}
+
+        def dep_phase(s):
+            # phase in which a value of s can first be computed
+            if s in memo:
+                return memo[s]
+            ph = 0
+            for part in s.parts:
+                for r in self._part_refs(part):
+                    if not self.in_group(r.objectRef):
+                        continue
+                    p = self.stage_of(r.objectRef)
+                    if p is s:
+                        raise PlanError("self reference in %s" % s.name)
+                    if p.inline:
+                        ph = max(ph, dep_phase(p))
+                    else:
+                        ph = max(ph, dep_phase(p) + 1)
+            memo[s] = ph
+            return ph
+        for s in self.stages:
+            if not s.inline:
+                s.phase = dep_phase(s)
+        self.nphases = 1 + max(s.phase for s in self.stages if not s.inline)
+
+        # phases in which an inline stage gets evaluated
+        def eval_phases(s):
+            out = set()
+            for (c, part, r) in s.consumers:
+                if c.inline:
+                    out |= eval_phases(c)
+                else:
+                    out.add(c.phase)
+            return out
+        self.eval_phases = {}
+        for s in self.stages:
+            if s.inline:
+                self.eval_phases[s] = eval_phases(s)
+        for s in self.stages:
+            if s.inline:
+                continue
+            reads = set()
+            for (c, part, r) in s.consumers:
+                reads |= (eval_phases(c) if c.inline else {c.phase})
+            s.needs_lds = bool(reads)
+            s.last_read = max(reads) if reads else s.phase
+
+    def _storage(self):
+        # storage classes: (element type, static extents); reuse a freed
+        # buffer of the same class (storage_mapping.py:387-459)
+        free = {}
+        buffers = []     # (offset, bytes, class)
+        offset = 0
+        order = sorted([s for s in self.stages if s.needs_lds],
+                       key=lambda s: (s.phase, s.index))
+        live = []        # (last_read, stage)
+        for s in order:
+            # release buffers whose last read is before this phase
+            still = []
+            for (lr, t) in live:
+                if lr < s.phase:
+                    free.setdefault(t.lds_class, []).append(t.lds_offset)
+                else:
+                    still.append((lr, t))
+            live = still
+            nbytes = s.func.typ.nbytes()
+            for e in s.static_ext:
+                nbytes *= e
+            cls = (s.func.typ.c_name(), tuple(s.static_ext))
+            s.lds_class = cls
+            if free.get(cls):
+                s.lds_offset = free[cls].pop(0)
+            else:
+                align = 16
+                offset = (offset + align - 1) // align * align
+                s.lds_offset = offset
+                offset += nbytes
+            live.append((s.last_read, s))
+        self.lds_bytes = (offset + 15) // 16 * 16
+        if self.lds_bytes > LDS_LIMIT_BYTES:
+            raise PlanError("group %d needs %d bytes of LDS (> %d); use smaller tiles"
+                            % (self.gindex, self.lds_bytes, LDS_LIMIT_BYTES))
+
+    # -- reporting ------------------------------------------------------------------
+    def describe(self):
+        lines = ['group %d: anchor %s, grid dims %s, tiles %s, LDS %d B' % (
+            self.gindex, self.anchor.name, self.grid_dims, self.tile_counts, self.lds_bytes)]
+        for s in self.stages:
+            kind = 'inline' if s.inline else 'phase %d%s' % (s.phase, ' lds@%d' % s.lds_offset
+                                                           if s.needs_lds else '')
+            lines.append('  %-12s %-18s ext=%s%s' % (s.name, kind, s.static_ext,
+                                                   ' liveout' if s.liveout else ''))
+        return '\n'.join(lines)

@@ -1,0 +1,164 @@
+"""Python side of the runtime: build, cache, load and call pipelines.
+
+Flow for a Pipeline (the MI355X analogue of the reference's
+``generate_code`` -> ``g++`` -> ``ctypes.cdll`` path, tuner.py:322-350,
+525-558 and apps/python/{cpp_compiler,loader}.py):
+
+  1. ``pipe.generate_code()`` -> HIP kernels + C++ host stub;
+  2. kernels -> code object with hiprtc (``pmx_compile_hip`` in
+     libpmx_rt.so; works without a GPU), cached by source hash under
+     ``polymage_amd/_kcache``;
+  3. host stub (code object embedded with .incbin) -> ``lib<name>_<hash>.so``
+     with g++, linked to libpmx_rt.so and libamdhip64;
+  4. ``ctypes.CDLL`` -> ``pipeline_<name>`` (host pointers, reference ABI)
+     and ``pipeline_<name>_dev`` (device pointers + stream).
+
+Nothing here falls back to a CPU path: if libpmx_rt.so is missing the
+import raises.
+"""
+
+import ctypes
+import hashlib
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+INCLUDE = os.path.join(REPO, 'include')
+RT_LIB = os.path.join(HERE, 'libpmx_rt.so')
+CACHE = os.environ.get('PMX_KCACHE', os.path.join(HERE, '_kcache'))
+ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
+ARCH = os.environ.get('PMX_ARCH', 'gfx950')
+
+# Device code flags: no FP contraction (results must equal the reference
+# C semantics bit for bit), no packed-FP32 instructions.
+HIPRTC_OPTS = ['-O3', '-ffp-contract=off', '-std=c++17',
+               '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+
+_lock = threading.Lock()
+_rt = None
+
+
+class RuntimeMissing(RuntimeError):
+    pass
+
+
+def rt():
+    """Load libpmx_rt.so (raises if it was not built)."""
+    global _rt
+    if _rt is None:
+        if not os.path.exists(RT_LIB):
+            raise RuntimeMissing("libpmx_rt.so not built; run __graft_entry__.build()")
+        lib = ctypes.CDLL(RT_LIB, mode=ctypes.RTLD_GLOBAL)
+        lib.pmx_last_error.restype = ctypes.c_char_p
+        lib.pmx_compile_hip.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+        lib.pmx_arena_bytes.restype = ctypes.c_size_t
+        _rt = lib
+    return _rt
+
+
+def last_error():
+    return rt().pmx_last_error().decode(errors='replace')
+
+
+def check(rc, what='pmx'):
+    if rc != 0:
+        raise RuntimeError('%s failed (%d): %s' % (what, rc, last_error()))
+
+
+def compile_hip(src, out_path, name='kernels.hip', opts=None):
+    opts = HIPRTC_OPTS if opts is None else opts
+    arr = (ctypes.c_char_p * len(opts))(*[o.encode() for o in opts])
+    log = ctypes.create_string_buffer(1 << 16)
+    rc = rt().pmx_compile_hip(src.encode(), name.encode(), ARCH.encode(), arr, len(opts),
+                              out_path.encode(), log, len(log))
+    if rc != 0:
+        raise RuntimeError('hiprtc compile failed: %s\n%s' % (last_error(), log.value.decode(errors='replace')[-4000:]))
+    return log.value.decode(errors='replace')
+
+
+def host_compile(cpp, so):
+    cmd = ['g++', '-O2', '-std=c++17', '-fPIC', '-shared', '-D__HIP_PLATFORM_AMD__',
+           '-I' + INCLUDE, '-I' + os.path.join(ROCM, 'include'), cpp, '-o', so,
+           '-L' + HERE, '-lpmx_rt', '-L' + os.path.join(ROCM, 'lib'), '-lamdhip64',
+           '-Wl,-rpath,$ORIGIN/..', '-Wl,-rpath,$ORIGIN/../..', '-Wl,-rpath,' + os.path.join(ROCM, 'lib'),
+           '-w']
+    subprocess.check_call(cmd)
+
+
+class BuiltPipeline(object):
+    """A compiled pipeline: ctypes handles to both entry points."""
+
+    def __init__(self, pipe, code, so_path):
+        self.pipe = pipe
+        self.code = code
+        self.so_path = so_path
+        rt()
+        self.lib = ctypes.CDLL(so_path)
+        self.name = pipe.name
+        self.params = pipe.sorted_parameters()
+        self.inputs = pipe.sorted_inputs()
+        self.outputs = pipe.sorted_outputs()
+        self.fn_host = self.lib['pipeline_' + self.name]
+        self.fn_host.restype = None
+        self.fn_host_rc = self.lib['pipeline_%s_host' % self.name]
+        self.fn_host_rc.restype = ctypes.c_int
+        self.fn_dev = self.lib['pipeline_%s_dev' % self.name]
+        self.fn_dev.restype = ctypes.c_int
+
+    def _param_args(self, values):
+        from .utils import convert_to_ctype
+        return [convert_to_ctype(p.typ.c_type_name(), values[p.name]) for p in self.params]
+
+    def run_host(self, values):
+        """Reference-style call on host ndarrays (synchronous)."""
+        args = self._param_args(values)
+        keep = []
+        for f in self.inputs + self.outputs:
+            a = values[f.name]
+            assert isinstance(a, np.ndarray) and a.flags['C_CONTIGUOUS'], f.name
+            assert a.dtype == f.typ.np_dtype(), (f.name, a.dtype)
+            keep.append(a)
+            args.append(ctypes.c_void_p(a.ctypes.data))
+        check(self.fn_host_rc(*args), 'pipeline_%s' % self.name)
+
+    def run_dev(self, values, stream=0):
+        """Device call: values[name] are device pointers (ints) or objects
+        with ``data_ptr()``; ``stream`` is a hipStream_t as int."""
+        args = self._param_args(values)
+        for f in self.inputs + self.outputs:
+            v = values[f.name]
+            ptr = v.data_ptr() if hasattr(v, 'data_ptr') else int(v)
+            args.append(ctypes.c_void_p(ptr))
+        args.append(ctypes.c_void_p(stream))
+        check(self.fn_dev(*args), 'pipeline_%s_dev' % self.name)
+
+
+def build(pipe, verbose=False):
+    """Generate, compile (cached) and load a pipeline."""
+    code = pipe.generate_code()
+    key = hashlib.sha256((code.hip_src + '\0' + code.host_src + '\0' +
+                          ' '.join(HIPRTC_OPTS) + ARCH).encode()).hexdigest()[:20]
+    d = os.path.join(CACHE, '%s_%s' % (pipe.name, key))
+    so = os.path.join(d, 'lib%s.so' % pipe.name)
+    with _lock:
+        if not os.path.exists(so):
+            os.makedirs(d, exist_ok=True)
+            hip = os.path.join(d, 'kernels.hip')
+            co = os.path.join(d, 'kernels.co')
+            with open(hip, 'w') as f:
+                f.write(code.hip_src)
+            if not os.path.exists(co):
+                compile_hip(code.hip_src, co)
+            cpp = os.path.join(d, 'stub.cpp')
+            with open(cpp, 'w') as f:
+                f.write(code.host_src.replace('@CO_PATH@', co))
+            tmp = so + '.tmp%d' % os.getpid()
+            host_compile(cpp, tmp)
+            os.replace(tmp, so)
+    return BuiltPipeline(pipe, code, so)

@@ -1,0 +1,42 @@
+"""Shared helpers for the test-suite: building pipelines at given sizes
+and running them on the GPU and through the oracle."""
+
+import numpy as np
+
+from polymage_amd import buildPipeline
+from polymage_amd.constructs import Condition
+
+
+def build(out, pd, sizes, name, tile_sizes, group_size=11, literal_mode='double',
+          gpu_options=None, params=None):
+    est = [(pd[k], v) for k, v in sizes.items()]
+    cons = [Condition(pd[k], '==', v) for k, v in sizes.items()]
+    return buildPipeline([out], param_estimates=est, param_constraints=cons,
+                         tile_sizes=tile_sizes, group_size=group_size, pipe_name=name,
+                         literal_mode=literal_mode, gpu_options=gpu_options)
+
+
+def bits_equal(a, b):
+    if a.dtype.kind == 'f':
+        return np.array_equal(a.view(np.uint32 if a.dtype == np.float32 else np.uint64),
+                              b.view(np.uint32 if b.dtype == np.float32 else np.uint64))
+    return np.array_equal(a, b)
+
+
+def max_rel(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    den = np.maximum(np.abs(b), 1e-30)
+    return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
+
+
+def gpu_build(pipe):
+    """runtime.build, or, with PMX_PREBUILD=1 (CPU-only cache warming),
+    build and skip the GPU part of the test."""
+    import os
+    import pytest
+    from polymage_amd import runtime
+    bp = runtime.build(pipe)
+    if os.environ.get('PMX_PREBUILD'):
+        pytest.skip("prebuild only")
+    return bp
