@@ -30,7 +30,7 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=5)
-    ap.add_argument('--workload', default='unsharp', choices=['unsharp', 'harris'])
+    ap.add_argument('--workload', default='unsharp', choices=['unsharp', 'harris', 'bilateral'])
     ap.add_argument('--tiles', default=None, help='e.g. 32x256')
     ap.add_argument('--threads', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')

@@ -259,7 +259,8 @@ class _Emitter(object):
                     tgt = red.accumulate_ref
                     vmap = dict(names)
                     for v, a in zip(f.variables, tgt.arguments):
-                        vmap[v] = '(' + self.expr(a, names) + ')'
+                        if v not in f.reductionVariables:
+                            vmap[v] = '(' + self.expr(a, names) + ')'
                     pro = []
                     tidx = []
                     for d, a in enumerate(tgt.arguments):

@@ -447,3 +447,33 @@ def pred_refs(pred):
     for c in pred:
         out += c.collect(Reference)
     return out
+
+
+def unconditional_refs(expr):
+    """References evaluated whatever the data: those not inside a Select
+    branch (Select conditions and everything outside Selects count)."""
+    from .expr_ast import (AbstractBinaryOpNode, AbstractUnaryOpNode, Cast,
+                           InbuiltFunction)
+    out = []
+
+    def walk(e):
+        e = Value.numericToValue(e)
+        if isinstance(e, Reference):
+            out.append(e)
+            for a in e.arguments:
+                walk(a)
+        elif isinstance(e, Select):
+            walk_cond(e.condition)
+        elif isinstance(e, (AbstractBinaryOpNode, AbstractUnaryOpNode, Cast, InbuiltFunction)):
+            for ch in e.children():
+                walk(ch)
+
+    def walk_cond(c):
+        if c.conditional in ('&&', '||'):
+            walk_cond(c.lhs)
+            walk_cond(c.rhs)
+        else:
+            walk(c.lhs)
+            walk(c.rhs)
+    walk(expr)
+    return out

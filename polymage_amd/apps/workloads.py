@@ -66,11 +66,25 @@ def unsharp(rows=2848, cols=4256, tiles=(32, 256), gpu_options=None, with_data=T
     return Workload('unsharp', pipe, out, vals, (3, R, C), out_px / 1e6, ideal)
 
 
-DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256)}
+def bilateral(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
+              group_size=5):
+    from .bilateral import bilateral_grid_full
+    pd = {}
+    out = bilateral_grid_full(pd)
+    pipe = _pipe(out, pd, {'rows': rows, 'cols': cols}, 'bilateral', list(tiles), group_size,
+                 gpu_options=gpu_options)
+    vals = {'rows': rows, 'cols': cols}
+    if with_data:
+        vals['input'] = inputs.bilateral_input(rows, cols)
+    ideal = rows * cols * 4 * 2
+    return Workload('bilateral', pipe, out, vals, (1, rows, cols), rows * cols / 1e6, ideal)
+
+
+DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64)}
 
 
 def prebuild_default():
     from .. import runtime
-    for fn in (harris, unsharp):
+    for fn in (harris, unsharp, bilateral):
         w = fn(with_data=False)
         runtime.build(w.pipe)

@@ -49,11 +49,12 @@ class Printer(object):
     """Prints expressions; ``ref_cb(ref, printer, prologue)`` returns the C
     text of a Reference, ``var_names`` maps Variables to C names."""
 
-    def __init__(self, var_names, ref_cb, mode='double', param_prefix=''):
+    def __init__(self, var_names, ref_cb, mode='double', param_prefix='', lazy=False):
         self.var_names = var_names
         self.ref_cb = ref_cb
         self.mode = mode
         self.param_prefix = param_prefix
+        self.lazy = lazy
         self._tmp = [0]
 
     def tmp(self):
@@ -76,6 +77,16 @@ class Printer(object):
         if isinstance(e, Reference):
             return self.ref_cb(e, self, pro)
         if isinstance(e, Select):
+            if self.lazy and pro is not None:
+                # Same value as the reference's "both branches into typed
+                # temporaries, then select", but only the taken branch runs,
+                # so an untaken branch never reads outside its producer.
+                c = self.cond(e.condition, pro)
+                t = self.expr(e.true_expression, pro)
+                f = self.expr(e.false_expression, pro)
+                return '((%s)((%s) ? (%s)(%s) : (%s)(%s)))' % (
+                    c_type(getType(e)), c, c_type(getType(e.true_expression)), t,
+                    c_type(getType(e.false_expression)), f)
             c = self.cond(e.condition, pro)
             t = self.expr(e.true_expression, pro)
             f = self.expr(e.false_expression, pro)

@@ -22,7 +22,7 @@ reference, and ``pipeline_<name>_dev(..., stream)`` on device pointers.
 
 import hashlib
 
-from .analysis import PlanError, classify_access, eval_bexpr
+from .analysis import PlanError, classify_access, eval_bexpr, unconditional_refs
 from .constructs import Reference, Image, Case, Reduction
 from .cprint import Printer, DEVICE_PRELUDE, c_type
 from .group_plan import GroupPlan
@@ -177,6 +177,18 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             e = term if e is None else '(%s * %d + %s)' % (e, p.static_ext[d], term)
         return e if e is not None else '0'
 
+    # references that only run inside a Select branch: their global reads
+    # are clamped to the producer's domain (the run-time bounds check in
+    # the host stub covers unconditional references only)
+    uncond = set()
+    for s in stages:
+        for part in s.parts:
+            for r in unconditional_refs(part.expr):
+                uncond.add(id(r))
+            for c in part.pred:
+                for r in c.collect(Reference):
+                    uncond.add(id(r))
+
     def make_ref_cb(cur_stage):
         def cb(ref, pr, pro):
             f = ref.objectRef
@@ -185,7 +197,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             for d, arg in enumerate(ref.arguments):
                 acc = classify_access(arg, cvars)
                 q = pr.expr(arg, pro)
-                if acc.kind == '*':
+                if acc.kind == '*' or (id(ref) not in uncond and not plan.in_group(f)):
                     if plan.in_group(f):
                         p = plan.stage_of(f)
                         q = 'pmx_clampi(%s, %s, %s)' % (q, R(p.region[d][0]), R(p.region[d][1]))
@@ -238,7 +250,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         """Statements assigning the value of stage s at point pnames to
         ``target``; returns name of a bool flag set when a part applied."""
         vn = dict(zip(s.func.variables, pnames))
-        pr = Printer(vn, make_ref_cb(s), mode)
+        pr = Printer(vn, make_ref_cb(s), mode, lazy=True)
         pr._tmp = tmp_counter
         single = (len(s.parts) == 1)
         for pi, part in enumerate(s.parts):

@@ -20,7 +20,7 @@ the way the reference's bounds check does (bounds.py:36-118), at run time,
 before anything touches the GPU.
 """
 
-from .analysis import classify_access
+from .analysis import classify_access, unconditional_refs
 from .constructs import Reference, Image, Reduction, Case, Reduce
 from .cprint import Printer, c_type
 from .region_ir import bound_c
@@ -57,7 +57,7 @@ def _validation_lines(pipe, pr):
                     hi = 'std::min<long long>(%s, %s)' % (hi, bound_c(bhi, pr))
                 box.append((lo, hi))
             nonempty = ' && '.join('(%s) <= (%s)' % b for b in box) or 'true'
-            refs = part.expr.collect(Reference)
+            refs = unconditional_refs(part.expr)
             for c in part.pred:
                 refs += c.collect(Reference)
             checks = []

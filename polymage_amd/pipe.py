@@ -27,7 +27,7 @@ from .constructs import (Function, Image, Reduction, Case, Reduce,
 from .expr_types import Int
 from .expression import substitute_vars, substitute_refs
 from .analysis import (PlanError, case_parts, classify_access, ceval_int,
-                       eval_bexpr, interval_image)
+                       eval_bexpr, interval_image, unconditional_refs)
 from .liveness import compute_liveness, level_order
 
 LOG = logging.getLogger("polymage_amd.pipe")
@@ -369,7 +369,10 @@ class Pipeline(object):
                     box.append((lo, hi))
                 if any(lo > hi for lo, hi in box):
                     continue
-                refs = part.expr.collect(Reference)
+                # references inside Select branches only run when their
+                # branch is taken; the GPU evaluates Selects lazily, so only
+                # unconditional references must stay inside the domain
+                refs = unconditional_refs(part.expr)
                 for c in part.pred:
                     refs += c.collect(Reference)
                 for r in refs:

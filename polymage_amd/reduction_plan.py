@@ -166,7 +166,7 @@ def emit_const_kernel(pipe, plan, kname, mode):
             t = '(%s - (%s))' % (q, base.expr(g.domain[d].lowerBound))
             lin = t if lin is None else '(%s * %s + %s)' % (lin, _extent_c(g, d, base), t)
         return 'G_%s[%s]' % (g.name, lin or '0')
-    pr = Printer(dict(zip(f.variables, names)), cb, mode)
+    pr = Printer(dict(zip(f.variables, names)), cb, mode, lazy=True)
     args = ['const %s %s' % (c_type(p.typ), p.name) for p in params]
     for g in buffers:
         if g is f:
@@ -206,17 +206,300 @@ def emit_const_kernel(pipe, plan, kname, mode):
                                            [g for g in buffers if g is not f], [f])
 
 
+class _RedCase(object):
+    def __init__(self, cond, red, index):
+        self.cond = cond
+        self.red = red
+        self.index = index
+
+
 class ReductionPlan(object):
-    """See module docstring."""
+    """See module docstring.  ``owner`` is True when the owner-cell kernel
+    applies; ``owned`` maps each reduction variable to the target dimension
+    it owns."""
+
+    MAX_ACC = 64
 
     def __init__(self, pipe, group, gindex):
         self.pipe = pipe
         self.group = group
         self.gindex = gindex
         self.comp = group.comps[0]
-        self.func = self.comp.func
-        raise PlanError("reductions are planned by ReductionPlan.build (not yet available)")
+        f = self.func = self.comp.func
+        self.rvars = list(f.reductionVariables)
+        self.fvars = list(f.variables)
+        self.cases = []
+        for i, case in enumerate(f.defn):
+            if isinstance(case, Case):
+                self.cases.append(_RedCase(case.condition, case.expression, i))
+            else:
+                self.cases.append(_RedCase(None, case, i))
+        for rc in self.cases:
+            if not isinstance(rc.red, Reduce):
+                raise PlanError("reduction %s: definition must be Reduce" % f.name)
+        self._classify()
+
+    def _classify(self):
+        f = self.func
+        nd = len(self.fvars)
+        owned = {}
+        ok = True
+        for j, rv in enumerate(self.rvars):
+            dim = None
+            for rc in self.cases:
+                tgt = rc.red.accumulate_ref.arguments
+                hit = [d for d in range(nd)
+                       if classify_access(tgt[d], self.rvars).kind == 'var'
+                       and classify_access(tgt[d], self.rvars).var is rv]
+                if len(hit) != 1:
+                    ok = False
+                    break
+                if dim is None:
+                    dim = (hit[0], str(tgt[hit[0]]))
+                elif dim != (hit[0], str(tgt[hit[0]])):
+                    ok = False
+                    break
+            if not ok or dim is None:
+                ok = False
+                break
+            acc = classify_access(rc.red.accumulate_ref.arguments[dim[0]], self.rvars)
+            if not acc.increasing:
+                ok = False
+                break
+            owned[rv] = dim[0]
+        env = self.pipe.param_env
+        self.free_dims = [d for d in range(nd) if d not in owned.values()]
+        nacc = 1
+        for d in self.free_dims:
+            lo = ceval_int(f.domain[d].lowerBound, env)
+            hi = ceval_int(f.domain[d].upperBound, env)
+            nacc *= (hi - lo + 1)
+        self.nacc = nacc
+        self.owner = ok and len(owned) == len(self.rvars) and nacc <= self.MAX_ACC
+        self.owned = owned
+
+    def describe(self):
+        return 'group %d: reduction %s (%s, %d accumulators per cell)' % (
+            self.gindex, self.func.name, 'owner cells' if self.owner else 'serial', self.nacc)
+
+
+def _reads_of_reduction(plan):
+    reads = []
+    for rc in plan.cases:
+        refs = rc.red.expression.collect(Reference)
+        refs += [r for r in rc.red.accumulate_ref.collect(Reference) if r is not rc.red.accumulate_ref]
+        if rc.cond is not None:
+            refs += rc.cond.collect(Reference)
+        for r in refs:
+            if r.objectRef is not plan.func and r.objectRef not in reads:
+                reads.append(r.objectRef)
+    return reads
 
 
 def emit_reduction_kernel(pipe, plan, kname, mode):
-    raise PlanError("reduction emission unavailable")
+    from .hip_emit import KernelInfo
+    f = plan.func
+    params = pipe.sorted_parameters()
+    reads = _reads_of_reduction(plan)
+    buffers = sorted(reads, key=lambda g: g.name) + [f]
+    base = Printer({}, None, mode)
+    T = c_type(f.typ)
+    nd = len(plan.fvars)
+    rnames = ['r%d' % j for j in range(len(plan.rvars))]
+
+    def cb(ref, pr, pro):
+        g = ref.objectRef
+        lin = None
+        for d, arg in enumerate(ref.arguments):
+            q = pr.expr(arg, pro)
+            acc = classify_access(arg, plan.rvars)
+            if acc.kind == '*':
+                q = 'pmx_clampi(%s, %s, %s)' % (q, base.expr(g.domain[d].lowerBound),
+                                                base.expr(g.domain[d].upperBound))
+            t = '(%s - (%s))' % (q, base.expr(g.domain[d].lowerBound))
+            lin = t if lin is None else '(%s * %s + %s)' % (lin, _extent_c(g, d, base), t)
+        return 'G_%s[%s]' % (g.name, lin or '0')
+
+    args = ['const %s %s' % (c_type(p.typ), p.name) for p in params]
+    for g in buffers:
+        if g is f:
+            args.append('%s* __restrict__ G_%s' % (T, g.name))
+        else:
+            args.append('const %s* __restrict__ G_%s' % (c_type(g.typ), g.name))
+    L = []
+    w = lambda s, ind=1: L.append('  ' * ind + s)
+    exts = [_extent_c(f, d, base) for d in range(nd)]
+    lbs = [base.expr(f.domain[d].lowerBound) for d in range(nd)]
+
+    def red_body(ind, target_ok):
+        """Statements processing the current reduction point (names
+        r0..rk bound).  ``target_ok(dimvals)`` filters owned cells."""
+        pr = Printer(dict(zip(plan.rvars, rnames)), cb, mode, lazy=True)
+        for rc in plan.cases:
+            red = rc.red
+            tgt = red.accumulate_ref.arguments
+            w('{', ind)
+            pro = []
+            tnames = []
+            for d in range(nd):
+                q = pr.expr(tgt[d], pro)
+                if classify_access(tgt[d], plan.rvars).kind == '*':
+                    q = 'pmx_clampi(%s, %s, %s)' % (q, lbs[d], base.expr(f.domain[d].upperBound))
+                tnames.append(q)
+            val = pr.expr(red.expression, pro)
+            for l in pro:
+                w(l, ind + 1)
+            for d in range(nd):
+                w('const int q%d = %s;' % (d, tnames[d]), ind + 1)
+            conds = []
+            if rc.cond is not None:
+                vn = dict(zip(plan.rvars, rnames))
+                for v, d in zip(plan.fvars, range(nd)):
+                    if v not in plan.rvars:
+                        vn[v] = 'q%d' % d
+                cpr = Printer(vn, cb, mode, lazy=True)
+                cpro = []
+                cc = cpr.cond(rc.cond, cpro)
+                for l in cpro:
+                    w(l, ind + 1)
+                conds.append(cc)
+            extra = target_ok()
+            if extra:
+                conds.append(extra)
+            slot = acc_slot()
+            op = red.op_type
+            if op == Op.Sum:
+                upd = '%s = (%s + %s);' % (slot, slot, val)
+            elif op == Op.Mul:
+                upd = '%s = (%s * %s);' % (slot, slot, val)
+            elif op == Op.Min:
+                upd = '%s = ((%s < %s)? (%s)(%s): %s);' % (slot, val, slot, T, val, slot)
+            else:
+                upd = '%s = ((%s > %s)? (%s)(%s): %s);' % (slot, val, slot, T, val, slot)
+            w('if (%s) { %s }' % (' && '.join(conds) or 'true', upd), ind + 1)
+            w('}', ind)
+
+    if plan.owner:
+        # thread per owned cell; accumulators for the free dims in LDS
+        od = sorted(plan.owned.values())
+        free = plan.free_dims
+        nthreads = 256
+        w_ = None
+        L.append('extern "C" __global__ __launch_bounds__(%d) void %s(%s) {' % (nthreads, kname, ', '.join(args)))
+        w('__shared__ %s pmx_acc[%d * %d];' % (T, nthreads, plan.nacc))
+        w('%s* acc = pmx_acc + threadIdx.x;' % T)
+        ncell = ' * '.join('(long long)%s' % exts[d] for d in od)
+        w('const long long pmx_ncell = %s;' % ncell)
+        w('long long cell = (long long)blockIdx.x * %d + threadIdx.x;' % nthreads)
+        w('if (cell >= pmx_ncell) return;')
+        for d in reversed(od):
+            w('const int c%d = (int)(cell %% %s) + (%s); cell /= %s;' % (d, exts[d], lbs[d], exts[d]))
+        # free-dim strides in the accumulator array (stride nthreads)
+        fext = [exts[d] for d in free]
+
+        def acc_index(names):
+            e = None
+            for k, d in enumerate(free):
+                t = '(%s - (%s))' % (names[d], lbs[d])
+                e = t if e is None else '(%s * %s + %s)' % (e, fext[k], t)
+            return '(%s) * %d' % (e or '0', nthreads)
+        # init with the default value
+        dvn = {}
+        w('for (int a = 0; a < %d; ++a) {' % plan.nacc)
+        rem = 'a'
+        fn = {}
+        for k in reversed(range(len(free))):
+            d = free[k]
+            w('  const int f%d = (%s) %% %s + (%s);' % (d, rem, fext[k], lbs[d]))
+            w('  const int a%d = (%s) / %s;' % (d, rem, fext[k]))
+            rem = 'a%d' % d
+            fn[d] = 'f%d' % d
+        names = dict((d, 'c%d' % d) for d in od)
+        names.update(fn)
+        dpr = Printer(dict((plan.fvars[d], names[d]) for d in range(nd)), cb, mode, lazy=True)
+        w('  acc[a * %d] = %s;' % (nthreads, dpr.expr(f.default)))
+        w('}')
+        # pre-image of the cell along every owned reduction variable
+        for j, rv in enumerate(plan.rvars):
+            d = plan.owned[rv]
+            arg = plan.cases[0].red.accumulate_ref.arguments[d]
+            rlo = base.expr(f.reductionDomain[j].lowerBound)
+            rhi = base.expr(f.reductionDomain[j].upperBound)
+            fpr = Printer({rv: 'm'}, cb, mode, lazy=True)
+            fx = fpr.expr(arg)
+            # lo_j = first r with f(r) >= c ; hi_j = last r with f(r) <= c
+            w('int lo%d, hi%d;' % (j, j))
+            w('{ int a = %s, b = %s + 1;' % (rlo, rhi))
+            w('  while (a < b) { const int m = a + (b - a) / 2; if ((%s) >= c%d) b = m; else a = m + 1; }' % (fx, d))
+            w('  lo%d = a; }' % j)
+            w('{ int a = %s - 1, b = %s;' % (rlo, rhi))
+            w('  while (a < b) { const int m = a + (b - a + 1) / 2; if ((%s) <= c%d) a = m; else b = m - 1; }' % (fx, d))
+            w('  hi%d = a; }' % j)
+        ind = 1
+        for j in range(len(plan.rvars)):
+            w('for (int r%d = lo%d; r%d <= hi%d; ++r%d) {' % (j, j, j, j, j), ind)
+            ind += 1
+
+        def acc_slot_owner():
+            nm = dict((d, 'q%d' % d) for d in free)
+            return 'acc[%s]' % acc_index(nm)
+        acc_slot = acc_slot_owner
+
+        def owned_ok():
+            return ' && '.join('q%d == c%d' % (d, d) for d in od)
+        red_body(ind, owned_ok)
+        for j in reversed(range(len(plan.rvars))):
+            ind -= 1
+            w('}', ind)
+        # write out the cell's slice of the grid
+        w('for (int a = 0; a < %d; ++a) {' % plan.nacc)
+        rem = 'a'
+        for k in reversed(range(len(free))):
+            d = free[k]
+            w('  const int f%d = (%s) %% %s + (%s);' % (d, rem, fext[k], lbs[d]))
+            w('  const int a%d = (%s) / %s;' % (d, rem, fext[k]))
+            rem = 'a%d' % d
+        lin = None
+        for d in range(nd):
+            t = '(long long)(%s - (%s))' % (names[d], lbs[d])
+            lin = t if lin is None else '(%s * %s + %s)' % (lin, exts[d], t)
+        w('  G_%s[%s] = acc[a * %d];' % (f.name, lin, nthreads))
+        w('}')
+        L.append('}')
+        grid = '((%s + %d) / %d)' % (ncell, nthreads - 1, nthreads)
+        return '\n'.join(L) + '\n', KernelInfo(kname, 'reduction', buffers, grid, nthreads, 0,
+                                               plan, reads, [f])
+
+    # serial fallback: one thread, the reference loop order
+    L.append('extern "C" __global__ __launch_bounds__(64) void %s(%s) {' % (kname, ', '.join(args)))
+    w('if (threadIdx.x != 0 || blockIdx.x != 0) return;')
+    lin = None
+    ind = 1
+    for d in range(nd):
+        w('for (int f%d = %s; f%d <= %s; ++f%d) {' % (d, lbs[d], d, base.expr(f.domain[d].upperBound), d), ind)
+        ind += 1
+        t = '(long long)(f%d - (%s))' % (d, lbs[d])
+        lin = t if lin is None else '(%s * %s + %s)' % (lin, exts[d], t)
+    dpr = Printer(dict((plan.fvars[d], 'f%d' % d) for d in range(nd)), cb, mode, lazy=True)
+    w('G_%s[%s] = %s;' % (f.name, lin or '0', dpr.expr(f.default)), ind)
+    for d in reversed(range(nd)):
+        ind -= 1
+        w('}', ind)
+    for j in range(len(plan.rvars)):
+        w('for (int r%d = %s; r%d <= %s; ++r%d) {' % (
+            j, base.expr(f.reductionDomain[j].lowerBound), j,
+            base.expr(f.reductionDomain[j].upperBound), j), ind)
+        ind += 1
+    tl = None
+    for d in range(nd):
+        t = '(long long)(q%d - (%s))' % (d, lbs[d])
+        tl = t if tl is None else '(%s * %s + %s)' % (tl, exts[d], t)
+
+    def acc_slot():
+        return 'G_%s[%s]' % (f.name, tl or '0')
+    red_body(ind, lambda: None)
+    for j in reversed(range(len(plan.rvars))):
+        ind -= 1
+        w('}', ind)
+    L.append('}')
+    return '\n'.join(L) + '\n', KernelInfo(kname, 'reduction', buffers, '1', 64, 0, plan, reads, [f])

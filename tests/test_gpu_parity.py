@@ -72,12 +72,42 @@ def test_unsharp_full_small_bitexact(gpu, R, C, tiles):
 
 
 def test_unsharp_app_defined_region(gpu):
-    # the reference app leaves y in {2, C-1, C} undefined (its Case quirk)
+    # the reference app leaves y in {2, C-1, C} undefined (its Case quirk);
+    # y index 0 is y = 2, so the defined columns are indices 1 .. C-4
     g, o = _unsharp(unsharp_mask, 64, 200, [16, 64])
-    assert bits_equal(g[:, :, 1:-2], o[:, :, 1:-2])
+    assert bits_equal(g[:, :, 1:-3], o[:, :, 1:-3])
 
 
 def test_unsharp_full_size(gpu):
     print("case unsharp 2844x4252", flush=True)
     g, o = _unsharp(unsharp_mask_full, 2844, 4252, [32, 256], seed=1002)
+    assert bits_equal(g, o)
+
+
+def _bilateral(rows, cols, tiles, seed=5, app=False):
+    from polymage_amd.apps.bilateral import bilateral_grid, bilateral_grid_full
+    pd = {}
+    out = (bilateral_grid if app else bilateral_grid_full)(pd)
+    pipe = build(out, pd, {'rows': rows, 'cols': cols}, 'bilateral', tiles, group_size=5)
+    img = inputs.smooth_field(np.random.default_rng(seed), rows, cols, 1)[0]
+    g = np.full((1, rows, cols), -7.0, np.float32)
+    o = np.full((1, rows, cols), -7.0, np.float32)
+    orc = cemit.build([out], 'bilateral_oracle_%s' % ('app' if app else 'full'))
+    gpu_build(pipe).run_host({'rows': rows, 'cols': cols, 'input': img, 'filtered': g})
+    orc({'rows': rows, 'cols': cols, 'input': img, 'filtered': o})
+    return g, o
+
+
+@pytest.mark.parametrize("rows,cols,tiles", [(64, 48, [16, 16]), (256, 192, [32, 64]),
+                                             (100, 77, [16, 32])])
+def test_bilateral_bitexact(gpu, rows, cols, tiles):
+    print("case bilateral", rows, cols, tiles, flush=True)
+    g, o = _bilateral(rows, cols, tiles)
+    assert np.all(np.isfinite(o))
+    assert bits_equal(g, o)
+
+
+def test_bilateral_full_size(gpu):
+    print("case bilateral 2560x1536", flush=True)
+    g, o = _bilateral(2560, 1536, [32, 64], seed=1003)
     assert bits_equal(g, o)
