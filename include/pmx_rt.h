@@ -47,6 +47,8 @@ int pmx_device_name(int dev, char* buf, size_t cap);
  * per-device buffer that only grows; it is kept until pmx_arena_release().
  * Not thread safe (like the reference pool, simple_pool_allocator.cpp:17). */
 int pmx_arena_get(const char* slot, size_t bytes, void** ptr);
+/* Same, but the slot is zero-filled when it is (re)allocated. */
+int pmx_arena_get_zeroed(const char* slot, size_t bytes, void** ptr);
 int pmx_arena_release(void);
 size_t pmx_arena_bytes(void);
 

@@ -103,6 +103,39 @@ def ceval(expr, env):
     raise PlanError("cannot evaluate %s statically" % (expr,))
 
 
+def ceval_np(expr, env):
+    """``ceval`` over numpy int64 arrays (C truncating ``/`` and ``%``)."""
+    import numpy as np
+    expr = Value.numericToValue(expr)
+    if isinstance(expr, Value):
+        return expr.value
+    if isinstance(expr, Variable):
+        if expr not in env:
+            raise PlanError("no value for %s" % expr.name)
+        return env[expr]
+    if isinstance(expr, AbstractBinaryOpNode):
+        l = ceval_np(expr.left, env)
+        r = ceval_np(expr.right, env)
+        op = expr.op
+        if op == '+':
+            return l + r
+        if op == '-':
+            return l - r
+        if op == '*':
+            return l * r
+        if op in ('/', '%'):
+            l = np.asarray(l, dtype=np.int64)
+            r = np.asarray(r, dtype=np.int64)
+            q = np.abs(l) // np.abs(r)
+            q = np.where((l >= 0) == (r >= 0), q, -q)
+            return q if op == '/' else l - r * q
+        raise PlanError(op)
+    if isinstance(expr, AbstractUnaryOpNode):
+        v = ceval_np(expr.child, env)
+        return -v if expr.op == '-' else v
+    raise PlanError("cannot evaluate %s statically" % (expr,))
+
+
 def ceval_int(expr, env):
     v = ceval(expr, env)
     if isinstance(v, Fraction):

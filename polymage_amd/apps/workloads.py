@@ -35,12 +35,14 @@ def _pipe(out, pd, sizes, name, tiles, group_size, literal_mode='double', gpu_op
                          literal_mode=literal_mode, gpu_options=gpu_options)
 
 
-def harris(rows=2048, cols=2048, tiles=(32, 256), gpu_options=None, with_data=True):
+def harris(rows=2048, cols=2048, tiles=(32, 256), gpu_options=None, with_data=True,
+           group_size=11):
     from .harris import harris_pipe
     pd = {}
     out = harris_pipe(pd)
     R, C = rows - 2, cols - 2
-    pipe = _pipe(out, pd, {'R': R, 'C': C}, 'harris', list(tiles), 11, gpu_options=gpu_options)
+    pipe = _pipe(out, pd, {'R': R, 'C': C}, 'harris', list(tiles), group_size,
+                 gpu_options=gpu_options)
     vals = {'R': R, 'C': C}
     if with_data:
         vals['img'] = inputs.harris_input(rows, cols)
@@ -80,7 +82,48 @@ def bilateral(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
     return Workload('bilateral', pipe, out, vals, (1, rows, cols), rows * cols / 1e6, ideal)
 
 
-DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64)}
+def laplacian(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
+              group_size=5, K=8, J=4):
+    """Local Laplacian, uint16 RGB, K intensity levels, J pyramid levels,
+    alpha = 1/(K-1), beta = 1; the input is padded by edge replication
+    inside the pipeline so the output has the input's size."""
+    from .local_laplacian import local_laplacian
+    pd = {}
+    out = local_laplacian(pd, K=K, J=J, io='u16', lut_hi_inclusive=True)
+    lo, hi = pd['pad']
+    R, C = rows + lo + hi, cols + lo + hi
+    pipe = _pipe(out, pd, {'R': R, 'C': C}, 'laplacian', list(tiles), group_size,
+                 gpu_options=gpu_options)
+    vals = {'R': R, 'C': C, 'alpha': 1.0 / (K - 1), 'beta': 1.0}
+    if with_data:
+        vals['img_colour'] = inputs.laplacian_input(rows, cols)
+    ideal = rows * cols * 3 * 2 * 2
+    return Workload('laplacian', pipe, out, vals, (rows, cols, 3), rows * cols / 1e6, ideal,
+                    dtype='u16')
+
+
+def campipe(rows=1216, cols=1664, tiles=(16, 64), gpu_options=None, with_data=True,
+            group_size=40):
+    """One frame of the batched camera pipeline (uint16 10-bit GRBG Bayer,
+    3700 K, gamma 2.2, contrast 50, 3x3 sharpen, uint8 RGB out)."""
+    from .campipe import camera_pipe, MATRIX_3200, MATRIX_7000
+    pd = {}
+    out = camera_pipe(pd, variant='batch')
+    pipe = _pipe(out, pd, {'R': rows, 'C': cols}, 'campipe', list(tiles), group_size,
+                 gpu_options=gpu_options)
+    vals = {'R': rows, 'C': cols, 'colour_temp': 3700.0, 'gamma': 2.2, 'contrast': 50.0,
+            'matrix_3200': np.array(MATRIX_3200, np.float32),
+            'matrix_7000': np.array(MATRIX_7000, np.float32)}
+    if with_data:
+        vals['img'] = inputs.rng(5).integers(0, 1024, (rows, cols)).astype(np.uint16)
+    opx = (rows - 48) * (cols - 64)
+    ideal = rows * cols * 2 + 3 * opx
+    return Workload('campipe', pipe, out, vals, (3, rows - 48, cols - 64), opx / 1e6, ideal,
+                    dtype='u8')
+
+
+DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64),
+                 'laplacian': (32, 64), 'campipe': (16, 64)}
 
 
 def prebuild_default():

@@ -98,7 +98,7 @@ int pmx_arena_get(const char* slot, size_t bytes, void** ptr) {
   std::lock_guard<std::mutex> lk(g_arena_mu);
   Slot& s = g_arena[{dev, std::string(slot)}];
   if (s.bytes < bytes) {
-    if (s.ptr) hipFree(s.ptr);
+    if (s.ptr) (void)hipFree(s.ptr);
     s.ptr = nullptr;
     s.bytes = 0;
     size_t want = (bytes + 4095) & ~size_t(4095);
@@ -110,10 +110,30 @@ int pmx_arena_get(const char* slot, size_t bytes, void** ptr) {
   return 0;
 }
 
+int pmx_arena_get_zeroed(const char* slot, size_t bytes, void** ptr) {
+  int dev = 0;
+  int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  bool fresh = false;
+  {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    auto it = g_arena.find({dev, std::string(slot)});
+    fresh = (it == g_arena.end() || it->second.bytes < bytes);
+  }
+  rc = pmx_arena_get(slot, bytes, ptr);
+  if (rc) return rc;
+  if (fresh) {
+    rc = hip_check(hipMemset(*ptr, 0, bytes), "hipMemset(arena)");
+    if (rc) return rc;
+    rc = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  }
+  return rc;
+}
+
 int pmx_arena_release(void) {
   std::lock_guard<std::mutex> lk(g_arena_mu);
   for (auto& kv : g_arena)
-    if (kv.second.ptr) hipFree(kv.second.ptr);
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   g_arena.clear();
   return 0;
 }
@@ -186,7 +206,7 @@ int pmx_graph_end(void* stream, void** graph_exec) {
   if (rc) return rc;
   hipGraphExec_t ex;
   rc = hip_check(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "hipGraphInstantiate");
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (rc) return rc;
   *graph_exec = (void*)ex;
   return 0;

@@ -333,23 +333,20 @@ class GroupPlan(object):
                 mid = n // 2
                 s = set(range(0, 8)) | set(range(n - 8, n)) | set(range(mid - 40, mid + 40))
                 per_dim.append(sorted(x for x in s if 0 <= x < n))
-        ext = dict((s, [0] * s.nd) for s in self.stages)
-        nsamp = 1
-        for p in per_dim:
-            nsamp *= max(1, len(p))
-        if nsamp > 20000:
-            # thin the sample uniformly
-            step = int(math.ceil((nsamp / 20000.0) ** (1.0 / max(1, len(per_dim)))))
-            per_dim = [p[::step] + ([p[-1]] if p and p[-1] not in p[::step] else []) for p in per_dim]
         dims = [d for (d, T) in self.grid_dims]
-        for combo in itertools.product(*per_dim):
-            tiles = dict(zip(dims, combo))
-            vals = self.ir.evaluate(self.env, tiles)
-            for s in self.stages:
-                for i, (l, h) in enumerate(s.region):
-                    e = vals[h] - vals[l] + 1
-                    if e > ext[s][i]:
-                        ext[s][i] = e
+        import numpy as np
+        if dims:
+            mesh = np.meshgrid(*[np.asarray(p, dtype=np.int64) for p in per_dim], indexing='ij')
+            tiles = dict((d, m.ravel()) for d, m in zip(dims, mesh))
+        else:
+            tiles = {}
+        vals = self.ir.evaluate_many(self.env, tiles)
+        ext = {}
+        for s in self.stages:
+            ext[s] = []
+            for (l, h) in s.region:
+                e = np.asarray(vals[h]) - np.asarray(vals[l]) + 1
+                ext[s].append(int(np.max(e)) if np.size(e) else 0)
         for s in self.stages:
             s.static_ext = [max(1, e) for e in ext[s]]
 

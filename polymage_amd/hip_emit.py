@@ -113,19 +113,9 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     L = []
     w = lambda s, ind=1: L.append('  ' * ind + s)
 
-    args = ['const %s %s' % (c_type(p.typ), p.name) for p in params]
-    for f in buffers:
-        if f in writes:
-            args.append('%s* __restrict__ %s' % (c_type(f.typ), _gname(f)))
-        else:
-            args.append('const %s* __restrict__ %s' % (c_type(f.typ), _gname(f)))
-    L.append('extern "C" __global__ __launch_bounds__(%d) void %s(%s) {' % (
-        threads, kname, ', '.join(args)))
-    if plan.lds_bytes:
-        w('__shared__ __align__(16) unsigned char pmx_lds[%d];' % plan.lds_bytes)
     # tile indices
     total = plan.total_tiles
-    w('int pmx_b = (int)blockIdx.x;')
+    w('int pmx_b = pmx_tile;')
     a = plan.anchor
     grid = plan.grid_dims
     # tile counts (runtime)
@@ -136,7 +126,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     if xcd:
         # blocks are dealt round-robin to the 8 XCDs; give each XCD a
         # contiguous run of tiles so neighbouring halos hit the same L2
-        w('{ const int per = pmx_ntiles / 8, rem = pmx_ntiles % 8;')
+        w('if (pmx_remap) { const int per = pmx_ntiles / 8, rem = pmx_ntiles % 8;')
         w('  const int xcd = pmx_b % 8, k = pmx_b / 8;')
         w('  pmx_b = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k; }')
     for (d, T) in reversed(grid):
@@ -213,6 +203,14 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             return '%s[%s]' % (_gname(f), gindex(f, idx))
         return cb
 
+    # a tile whose region exceeds the planned static box (possible only for
+    # sizes far from the planning estimates) is skipped and reported
+    over = []
+    for s_ in stages:
+        for i in range(s_.nd):
+            over.append('(%s - %s + 1 > %d)' % (R(s_.region[i][1]), R(s_.region[i][0]), s_.static_ext[i]))
+    if over:
+        w('if (%s) { if (threadIdx.x == 0) atomicOr(pmx_err, 1); return; }' % ' || '.join(over))
     # LDS typed views
     for s in stages:
         if s.needs_lds:
@@ -327,11 +325,73 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             w('}')
         if ph + 1 < plan.nphases:
             w('__syncthreads();')
-    L.append('}')
     grid_c = ' * '.join('((%s + %d) / %d)' % (_extent_c(a.func, d, base_pr), T - 1, T)
                         for (d, T) in grid) or '1'
-    return '\n'.join(L) + '\n', KernelInfo(kname, 'group', buffers, grid_c, threads,
-                                           plan.lds_bytes, plan, reads, writes)
+    info = KernelInfo(kname, 'group', buffers, grid_c, threads, plan.lds_bytes, plan, reads, writes)
+    return wrap_kernel(kname, params, buffers, writes, threads, plan.lds_bytes, L), info
+
+
+
+
+def kernel_args(params, buffers, writes):
+    """(declarations, names, frame-struct fields) of a kernel's arguments."""
+    decls, names, fields = ['int* __restrict__ pmx_err'], ['pmx_err'], ['int* pmx_err;']
+    for f in buffers:
+        q = '' if f in writes else 'const '
+        decls.append('%s%s* __restrict__ G_%s' % (q, c_type(f.typ), f.name))
+        names.append('G_' + f.name)
+        fields.append('%s%s* G_%s;' % (q, c_type(f.typ), f.name))
+    fields.append('long long pmx_tile0;')
+    for p in params:
+        decls.append('const %s %s' % (c_type(p.typ), p.name))
+        names.append(p.name)
+        fields.append('%s %s;' % (c_type(p.typ), p.name))
+    return decls, names, fields
+
+
+def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines):
+    """The kernel body as a device function plus two entry points:
+
+    ``<k>``       one workgroup per tile (grid = number of tiles);
+    ``<k>_batch`` persistent: a fixed set of workgroups pulls (frame, tile)
+                  work items from an atomic counter until the queue over all
+                  frames is empty; frames are described by ``<k>_frame``
+                  records (buffers, first global work item, parameters).
+    Every workgroup of the persistent form leaves the loop once the counter
+    passes ``total``, so the grid always drains."""
+    decls, names, fields = kernel_args(params, buffers, writes)
+    lds = max(16, lds_bytes)
+    L = ['__device__ __forceinline__ void %s_body(%s) {' % (
+        kname, ', '.join(decls + ['const int pmx_tile', 'const int pmx_remap',
+                                  'unsigned char* __restrict__ pmx_lds']))]
+    L += ['  (void)pmx_remap; (void)pmx_lds; (void)pmx_err;']
+    L += body_lines
+    L.append('}')
+    L.append('extern "C" __global__ __launch_bounds__(%d) void %s(%s) {' % (
+        threads, kname, ', '.join(decls)))
+    L.append('  __shared__ __align__(16) unsigned char pmx_lds[%d];' % lds)
+    L.append('  %s_body(%s);' % (kname, ', '.join(names + ['(int)blockIdx.x', '1', 'pmx_lds'])))
+    L.append('}')
+    L.append('struct %s_frame { %s };' % (kname, ' '.join(fields)))
+    L.append('extern "C" __global__ __launch_bounds__(%d) void %s_batch(const %s_frame* __restrict__ F, '
+             'const int nframes, unsigned long long* __restrict__ counter, const long long total) {'
+             % (threads, kname, kname))
+    L.append('  __shared__ __align__(16) unsigned char pmx_lds[%d];' % lds)
+    L.append('  __shared__ long long pmx_work;')
+    L.append('  for (;;) {')
+    L.append('    __syncthreads();')
+    L.append('    if (threadIdx.x == 0) pmx_work = (long long)atomicAdd(counter, 1ULL);')
+    L.append('    __syncthreads();')
+    L.append('    const long long t = pmx_work;')
+    L.append('    if (t >= total) break;')
+    L.append('    int lo = 0, hi = nframes - 1;')
+    L.append('    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (F[mid].pmx_tile0 <= t) lo = mid; else hi = mid - 1; }')
+    L.append('    const %s_frame& f = F[lo];' % kname)
+    L.append('    %s_body(%s);' % (kname, ', '.join(['f.' + n for n in names] +
+                                                   ['(int)(t - f.pmx_tile0)', '0', 'pmx_lds'])))
+    L.append('  }')
+    L.append('}')
+    return '\n'.join(L) + '\n'
 
 
 # ---------------------------------------------------------------------------
@@ -358,7 +418,10 @@ def emit_pipeline(pipe, is_extern_c_func=True, are_io_void_ptrs=True):
     xcd = bool(opts.get('xcd_swizzle', True))
     plans = plan_pipeline(pipe)
     srcs = ['// generated by polymage_amd for pipeline %s' % pipe.name,
-            DEVICE_PRELUDE, REGION_PRELUDE]
+            DEVICE_PRELUDE, REGION_PRELUDE,
+            'extern "C" __global__ void pmx_zero_u64(unsigned long long* p, const int n) {\n'
+            '  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);\n'
+            '  if (i < n) p[i] = 0ULL;\n}\n']
     kernels = []
     host_consts = []
     for i, plan in enumerate(plans):

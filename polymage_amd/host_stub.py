@@ -103,6 +103,8 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
     w('#include <cstring>')
     w('#include <climits>')
     w('#include <vector>')
+    w('#include <map>')
+    w('#include <set>')
     w('#include "pmx_rt.h"')
     w('extern "C" const unsigned char %s_image[];' % co_symbol)
     w('__asm__(".section .rodata\\n.balign 64\\n.global %s_image\\n%s_image:\\n.incbin \\"%s\\"\\n.byte 0\\n.previous\\n");'
@@ -111,12 +113,18 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
         max(1, nk), ', '.join('"%s"' % k.name for k in kernels) or '""'))
     w('static void* pmx_mod[64];')
     w('static void* pmx_fn[64][%d];' % max(1, nk))
+    w('static void* pmx_fn_batch[64][%d];' % max(1, nk))
+    w('static void* pmx_zero_fn[64];')
+    w('static char pmx_slot_buf[256];')
     w('static int pmx_load(int dev) {')
     w('  if (dev < 0 || dev >= 64) { pmx_set_error("device index"); return -1; }')
     w('  if (pmx_mod[dev]) return 0;')
     w('  void* m = nullptr;')
     w('  int rc = pmx_module_load_data(%s_image, &m); if (rc) return rc;' % co_symbol)
     w('  for (int k = 0; k < %d; ++k) { rc = pmx_module_function(m, pmx_kernel_names[k], &pmx_fn[dev][k]); if (rc) return rc; }' % nk)
+    w('  for (int k = 0; k < %d; ++k) { char nm[256]; snprintf(nm, sizeof nm, "%%s_batch", pmx_kernel_names[k]); '
+      'rc = pmx_module_function(m, nm, &pmx_fn_batch[dev][k]); if (rc) return rc; }' % nk)
+    w('  rc = pmx_module_function(m, "pmx_zero_u64", &pmx_zero_fn[dev]); if (rc) return rc;')
     w('  pmx_mod[dev] = m;')
     w('  return 0;')
     w('}')
@@ -149,6 +157,13 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
         bufname[f] = '(void*)%s_dev' % f.name
     for f in outputs:
         bufname[f] = '%s_dev' % f.name
+    for cp in host_consts:
+        f = cp.func
+        b = 'b_%s' % f.name
+        w('  void* %s = nullptr;' % b)
+        w('  rc = %s(%s, stream, &%s); if (rc) return rc;' % (
+            cp.upload_fn, ', '.join(p.name for p in cp.params), b))
+        bufname[f] = b
     for f in all_funcs:
         if f in bufname:
             continue
@@ -157,24 +172,16 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
         w('  rc = pmx_arena_get("%s/%s", (size_t)(%s) * sizeof(%s), &%s); if (rc) return rc;'
           % (name, f.name, _size(f, pr), c_type(f.typ), b))
         bufname[f] = b
-    for cp in host_consts:
-        f = cp.func
-        if f in bufname:
-            continue
-        b = 'b_%s' % f.name
-        w('  void* %s = nullptr;' % b)
-        w('  rc = %s(%s, stream, &%s); if (rc) return rc;' % (
-            cp.upload_fn, ', '.join(p.name for p in cp.params), b))
-        bufname[f] = b
+    w('  void* pmx_err = nullptr; rc = pmx_arena_get_zeroed("%s/err", 64, &pmx_err); if (rc) return rc;' % name)
     for k in kernels:
         w('  {')
-        argv = []
-        for p in params:
-            w('    %s a_%s = %s;' % (c_type(p.typ), p.name, p.name))
-            argv.append('&a_%s' % p.name)
+        argv = ['&pmx_err']
         for f in k.buffers:
             w('    void* a_%s = %s;' % (f.name, bufname[f]))
             argv.append('&a_%s' % f.name)
+        for p in params:
+            w('    %s a_%s = %s;' % (c_type(p.typ), p.name, p.name))
+            argv.append('&a_%s' % p.name)
         w('    void* args[] = {%s};' % (', '.join(argv) or 'nullptr'))
         w('    long long grid = %s;' % k.grid_c)
         w('    if (grid > 0) { rc = pmx_launch(pmx_fn[dev][%d], (unsigned)grid, 1, 1, %d, 1, 1, 0, stream, args); if (rc) return rc; }'
@@ -186,6 +193,14 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
     # host-pointer entry point (reference ABI)
     hsig = sig + ['void* %s_void_arg' % f.name for f in inputs] + \
         ['void* %s_void_arg' % f.name for f in outputs]
+    w('extern "C" int pipeline_%s_status(void) {' % name)
+    w('  void* e = nullptr; int rc = pmx_arena_get_zeroed("%s/err", 64, &e); if (rc) return rc;' % name)
+    w('  int v = 0; rc = pmx_memcpy_d2h(&v, e, sizeof v, nullptr); if (rc) return rc;')
+    w('  rc = pmx_stream_sync(nullptr); if (rc) return rc;')
+    w('  if (v) { int z = 0; pmx_memcpy_h2d(e, &z, sizeof z, nullptr); pmx_stream_sync(nullptr);')
+    w('    pmx_set_error("%s: a tile exceeded its planned LDS box; rebuild with estimates closer to these sizes"); return -34; }' % name)
+    w('  return 0;')
+    w('}')
     w('extern "C" int pipeline_%s_host(%s) {' % (name, ', '.join(hsig)))
     w('  int rc = 0;')
     for f in inputs + outputs:
@@ -199,11 +214,170 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
     for f in outputs:
         w('  rc = pmx_memcpy_d2h(%s_void_arg, d_%s, n_%s, nullptr); if (rc) return rc;' % (
             f.name, f.name, f.name))
-    w('  return pmx_stream_sync(nullptr);')
+    w('  rc = pmx_stream_sync(nullptr); if (rc) return rc;')
+    w('  return pipeline_%s_status();' % name)
     w('}')
+    _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None)
     w('extern "C" void pipeline_%s(%s) {' % (name, ', '.join(hsig)))
     w('  int rc = pipeline_%s_host(%s);' % (name, ', '.join(
         [p.name for p in params] + ['%s_void_arg' % f.name for f in inputs + outputs])))
     w('  if (rc) fprintf(stderr, "pipeline_%s: %%s\\n", pmx_last_error());' % name)
     w('}')
     return '\n'.join(L) + '\n'
+
+
+def frame_struct_fields(kernel, params):
+    """Field declarations of <kernel>_frame (same text as the device side,
+    see hip_emit.kernel_args) and the struct size on LP64 / amdgcn."""
+    from .hip_emit import kernel_args
+    decls, names, fields = kernel_args(params, kernel.buffers, kernel.writes)
+    size = 8 * len(kernel.buffers) + 16
+    for p in params:
+        size += p.typ.nbytes()
+    size = (size + 7) // 8 * 8
+    return fields, size
+
+
+def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
+    """Batched execution of many frames of different sizes.
+
+    pipeline_<name>_batch_create(n, per-frame parameter arrays...,
+        per-frame input pointer arrays..., per-frame output pointer arrays...,
+        &handle): plans every kernel's work items over all frames, allocates
+        per-frame intermediates, uploads the frame records.
+    pipeline_<name>_batch_run(handle, stream): one persistent work-queue
+        launch per kernel for all frames.
+    pipeline_<name>_batch_run_graph(handle, stream): the same sequence
+        captured once in a hipGraph and replayed.
+    pipeline_<name>_batch_run_frames(handle, stream): the per-frame plain
+        launches (pipeline_<name>_dev once per frame) for comparison.
+    pipeline_<name>_batch_destroy(handle)."""
+    w = L.append
+    name = pipe.name
+    params = pipe.sorted_parameters()
+    inputs = pipe.sorted_inputs()
+    outputs = pipe.sorted_outputs()
+    nk = len(kernels)
+    for k in kernels:
+        fields, size = frame_struct_fields(k, params)
+        w('struct %s_frame { %s };' % (k.name, ' '.join(fields)))
+        w('static_assert(sizeof(%s_frame) == %d, "frame record layout");' % (k.name, size))
+    w('struct pmx_batch_%s {' % name)
+    w('  int n = 0; int dev = 0;')
+    w('  void* frames[%d] = {};' % max(1, nk))
+    w('  long long total[%d] = {};' % max(1, nk))
+    w('  void* counters = nullptr; void* graph = nullptr;')
+    for p in params:
+        w('  std::vector<%s> p_%s;' % (c_type(p.typ), p.name))
+    for f in inputs + outputs:
+        w('  std::vector<void*> io_%s;' % f.name)
+    w('};')
+    sig = ['int n'] + ['const %s* %s_arr' % (c_type(p.typ), p.name) for p in params] + \
+        ['void* const* %s_arr' % f.name for f in inputs + outputs] + ['void** handle']
+    w('extern "C" int pipeline_%s_batch_create(%s) {' % (name, ', '.join(sig)))
+    w('  if (n <= 0) { pmx_set_error("batch: no frames"); return -22; }')
+    w('  pmx_batch_%s* B = new pmx_batch_%s(); B->n = n;' % (name, name))
+    w('  if (hipGetDevice(&B->dev) != hipSuccess) { delete B; return -1; }')
+    w('  int rc = pmx_load(B->dev); if (rc) { delete B; return rc; }')
+    for p in params:
+        w('  B->p_%s.assign(%s_arr, %s_arr + n);' % (p.name, p.name, p.name))
+    for f in inputs + outputs:
+        w('  B->io_%s.assign(%s_arr, %s_arr + n);' % (f.name, f.name, f.name))
+    for k in kernels:
+        w('  std::vector<%s_frame> rec_%s((size_t)n);' % (k.name, k.name))
+    w('  for (int i = 0; i < n; ++i) {')
+    for p in params:
+        w('    const %s %s = %s_arr[i];' % (c_type(p.typ), p.name, p.name))
+    for l in _validation_lines(pipe, pr):
+        w('    ' + l.replace('return -22;', 'delete B; return -22;'))
+    w('    char slot[256];')
+    bufname = {}
+    for f in inputs + outputs:
+        bufname[f] = '%s_arr[i]' % f.name
+    for cp in host_consts:
+        f = cp.func
+        b = 'b_%s' % f.name
+        w('    void* %s = nullptr;' % b)
+        w('    rc = %s(%s, nullptr, &%s); if (rc) { delete B; return rc; }' % (
+            cp.upload_fn, ', '.join(p.name for p in cp.params), b))
+        bufname[f] = b
+    for k in kernels:
+        for f in k.buffers:
+            if f in bufname:
+                continue
+            b = 'b_%s' % f.name
+            w('    void* %s = nullptr;' % b)
+            w('    snprintf(slot, sizeof slot, "%s/batch/%%d/%s", i);' % (name, f.name))
+            w('    rc = pmx_arena_get(slot, (size_t)(%s) * sizeof(%s), &%s); if (rc) { delete B; return rc; }'
+              % (_size(f, pr), c_type(f.typ), b))
+            bufname[f] = b
+    w('    void* pmx_err = nullptr; rc = pmx_arena_get_zeroed("%s/err", 64, &pmx_err); if (rc) { delete B; return rc; }' % name)
+    for k in kernels:
+        r = 'rec_%s[i]' % k.name
+        w('    %s.pmx_err = (int*)pmx_err;' % r)
+        for f in k.buffers:
+            ct = c_type(f.typ)
+            q = '' if f in k.writes else 'const '
+            w('    %s.G_%s = (%s%s*)%s;' % (r, f.name, q, ct, bufname[f]))
+        for p in params:
+            w('    %s.%s = %s;' % (r, p.name, p.name))
+        w('    %s.pmx_tile0 = (long long)(%s);' % (r, k.grid_c))
+    w('  }')
+    # prefix sums -> first work item of every frame
+    for ki, k in enumerate(kernels):
+        w('  { long long acc = 0; for (int i = 0; i < n; ++i) { long long c = rec_%s[i].pmx_tile0; '
+          'rec_%s[i].pmx_tile0 = acc; acc += c; } B->total[%d] = acc; }' % (k.name, k.name, ki))
+        w('  snprintf(pmx_slot_buf, sizeof pmx_slot_buf, "%s/batch/rec%d/%%p", (void*)B);' % (name, ki))
+        w('  rc = pmx_malloc(&B->frames[%d], sizeof(%s_frame) * (size_t)n); if (rc) { delete B; return rc; }'
+          % (ki, k.name))
+        w('  rc = pmx_memcpy_h2d(B->frames[%d], rec_%s.data(), sizeof(%s_frame) * (size_t)n, nullptr); '
+          'if (rc) { delete B; return rc; }' % (ki, k.name, k.name))
+    w('  rc = pmx_malloc(&B->counters, 8 * %d); if (rc) { delete B; return rc; }' % max(1, nk))
+    w('  rc = pmx_stream_sync(nullptr); if (rc) { delete B; return rc; }')
+    w('  *handle = (void*)B;')
+    w('  return 0;')
+    w('}')
+    w('extern "C" int pipeline_%s_batch_run(void* handle, void* stream) {' % name)
+    w('  pmx_batch_%s* B = (pmx_batch_%s*)handle;' % (name, name))
+    w('  int rc = pmx_load(B->dev); if (rc) return rc;')
+    w('  { void* c = B->counters; int nc = %d; void* args[] = {&c, &nc};' % max(1, nk))
+    w('    rc = pmx_launch(pmx_zero_fn[B->dev], 1, 1, 1, 64, 1, 1, 0, stream, args); if (rc) return rc; }')
+    for ki, k in enumerate(kernels):
+        w('  { void* f = B->frames[%d]; int n = B->n; unsigned long long* c = (unsigned long long*)B->counters + %d; '
+          'long long t = B->total[%d];' % (ki, ki, ki))
+        w('    void* args[] = {&f, &n, &c, &t};')
+        w('    long long g = t < %d ? t : %d;' % (PERSISTENT_WG, PERSISTENT_WG))
+        w('    if (g > 0) { rc = pmx_launch(pmx_fn_batch[B->dev][%d], (unsigned)g, 1, 1, %d, 1, 1, 0, stream, args); if (rc) return rc; } }'
+          % (ki, k.threads))
+    w('  return 0;')
+    w('}')
+    w('extern "C" int pipeline_%s_batch_run_graph(void* handle, void* stream) {' % name)
+    w('  pmx_batch_%s* B = (pmx_batch_%s*)handle;' % (name, name))
+    w('  int rc = 0;')
+    w('  if (!B->graph) {')
+    w('    rc = pmx_graph_begin(stream); if (rc) return rc;')
+    w('    int rc2 = pipeline_%s_batch_run(handle, stream);' % name)
+    w('    rc = pmx_graph_end(stream, &B->graph); if (rc2) return rc2; if (rc) return rc;')
+    w('  }')
+    w('  return pmx_graph_launch(B->graph, stream);')
+    w('}')
+    w('extern "C" int pipeline_%s_batch_run_frames(void* handle, void* stream) {' % name)
+    w('  pmx_batch_%s* B = (pmx_batch_%s*)handle;' % (name, name))
+    w('  for (int i = 0; i < B->n; ++i) {')
+    call = ['B->p_%s[i]' % p.name for p in params] + ['B->io_%s[i]' % f.name for f in inputs + outputs] + ['stream']
+    w('    int rc = pipeline_%s_dev(%s); if (rc) return rc;' % (name, ', '.join(call)))
+    w('  }')
+    w('  return 0;')
+    w('}')
+    w('extern "C" int pipeline_%s_batch_destroy(void* handle) {' % name)
+    w('  pmx_batch_%s* B = (pmx_batch_%s*)handle;' % (name, name))
+    w('  if (!B) return 0;')
+    w('  if (B->graph) pmx_graph_destroy(B->graph);')
+    w('  for (int k = 0; k < %d; ++k) if (B->frames[k]) pmx_free(B->frames[k]);' % max(1, nk))
+    w('  if (B->counters) pmx_free(B->counters);')
+    w('  delete B;')
+    w('  return 0;')
+    w('}')
+
+
+PERSISTENT_WG = 1024

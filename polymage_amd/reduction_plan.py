@@ -21,7 +21,7 @@
 from fractions import Fraction
 
 from .analysis import PlanError, classify_access, case_parts, ceval_int, eval_bexpr, interval_image
-from .constructs import Reference, Image, Case, Reduce, Op, Reduction
+from .constructs import Reference, Image, Case, Reduce, Op, Reduction, Parameter
 from .cprint import Printer, c_type
 from .expr_ast import Variable, Value
 from .region_ir import bound_c
@@ -86,19 +86,22 @@ class ConstPlan(object):
         w('static inline float pmx_cos(float v) { return std::cos(v); }')
         w('static inline double pmx_cos(double v) { return std::cos(v); }')
         sig = ', '.join('%s %s' % (c_type(p.typ), p.name) for p in self.params)
-        w('static std::vector<%s> pmx_tab_%s;' % (T, f.name))
-        w('static std::vector<unsigned char> pmx_tabkey_%s;' % f.name)
-        w('static bool pmx_tab_fresh_%s[64];' % f.name)
+        used = set(id(q) for q in f.getObjects(Parameter))
+        keyp = [p for p in self.params if id(p) in used]
+        w('static std::map<std::vector<unsigned char>, std::vector<%s> > pmx_tab_%s;' % (T, f.name))
+        w('static std::set<std::pair<int, std::vector<unsigned char> > > pmx_tab_up_%s;' % f.name)
         w('static int %s(%s%svoid* stream, void** out) {' % (self.upload_fn, sig, ', ' if sig else ''))
-        # key = raw bytes of the parameters
-        w('  std::vector<unsigned char> key;')
         for p in self.params:
+            w('  (void)%s;' % p.name)
+        w('  std::vector<unsigned char> key;')
+        for p in keyp:
             w('  { const unsigned char* q = (const unsigned char*)&%s; key.insert(key.end(), q, q + sizeof(%s)); }'
               % (p.name, p.name))
         size = ' * '.join('(long long)%s' % _extent_c(f, d, hp) for d in range(nd)) or '1'
         w('  long long n = %s;' % size)
-        w('  if (key != pmx_tabkey_%s || (long long)pmx_tab_%s.size() != n) {' % (f.name, f.name))
-        w('    pmx_tab_%s.assign((size_t)n, (%s)0);' % (f.name, T))
+        w('  std::vector<%s>& tab = pmx_tab_%s[key];' % (T, f.name))
+        w('  if ((long long)tab.size() != n) {')
+        w('    tab.assign((size_t)n, (%s)0);' % T)
         for i in range(nd):
             w('    ' + '  ' * i + 'for (int %s = %s; %s <= %s; ++%s) {' % (
                 names[i], hp.expr(f.domain[i].lowerBound), names[i],
@@ -121,21 +124,20 @@ class ConstPlan(object):
             w(ind + 'if (%s) {' % (' && '.join(conds) or 'true'))
             for l in pro:
                 w(ind + '  ' + l)
-            w(ind + '  if (%s) pmx_tab_%s[%s] = %s;' % (' && '.join(pc) or 'true', f.name, lin or '0', val))
+            w(ind + '  if (%s) tab[%s] = %s;' % (' && '.join(pc) or 'true', lin or '0', val))
             w(ind + '}')
         for i in reversed(range(nd)):
             w('    ' + '  ' * i + '}')
-        w('    pmx_tabkey_%s = key;' % f.name)
-        w('    for (int d = 0; d < 64; ++d) pmx_tab_fresh_%s[d] = false;' % f.name)
         w('  }')
         w('  int dev = 0; if (hipGetDevice(&dev) != hipSuccess) return -1;')
-        w('  int rc = pmx_arena_get("%s/const/%s", (size_t)n * sizeof(%s), out); if (rc) return rc;'
-          % (self.pipe.name, f.name, T))
-        w('  if (!pmx_tab_fresh_%s[dev]) {' % f.name)
-        w('    rc = pmx_memcpy_h2d(*out, pmx_tab_%s.data(), (size_t)n * sizeof(%s), stream); if (rc) return rc;'
-          % (f.name, T))
+        w('  unsigned long long h = 1469598103934665603ULL;')
+        w('  for (unsigned char b : key) { h ^= b; h *= 1099511628211ULL; }')
+        w('  char slot[128]; snprintf(slot, sizeof slot, "%s/const/%s/%%016llx", h);' % (self.pipe.name, f.name))
+        w('  int rc = pmx_arena_get(slot, (size_t)n * sizeof(%s), out); if (rc) return rc;' % T)
+        w('  if (!pmx_tab_up_%s.count(std::make_pair(dev, key))) {' % f.name)
+        w('    rc = pmx_memcpy_h2d(*out, tab.data(), (size_t)n * sizeof(%s), stream); if (rc) return rc;' % T)
         w('    rc = pmx_stream_sync(stream); if (rc) return rc;')
-        w('    pmx_tab_fresh_%s[dev] = true;' % f.name)
+        w('    pmx_tab_up_%s.insert(std::make_pair(dev, key));' % f.name)
         w('  }')
         w('  return 0;')
         w('}')
@@ -145,7 +147,7 @@ class ConstPlan(object):
 def emit_const_kernel(pipe, plan, kname, mode):
     """One thread per table point; the table reads images (e.g. a colour
     matrix interpolated from two input matrices)."""
-    from .hip_emit import KernelInfo
+    from .hip_emit import KernelInfo, wrap_kernel
     f = plan.func
     params = pipe.sorted_parameters()
     reads = []
@@ -173,11 +175,11 @@ def emit_const_kernel(pipe, plan, kname, mode):
             args.append('%s* __restrict__ G_%s' % (c_type(g.typ), g.name))
         else:
             args.append('const %s* __restrict__ G_%s' % (c_type(g.typ), g.name))
-    L = ['extern "C" __global__ __launch_bounds__(256) void %s(%s) {' % (kname, ', '.join(args))]
+    L = []
     w = lambda s: L.append('  ' + s)
     exts = [_extent_c(f, d, base) for d in range(nd)]
     w('long long pmx_n = %s;' % (' * '.join('(long long)%s' % e for e in exts) or '1'))
-    w('long long idx = (long long)blockIdx.x * 256 + threadIdx.x;')
+    w('long long idx = (long long)pmx_tile * 256 + threadIdx.x;')
     w('if (idx >= pmx_n) return;')
     w('long long rem = idx;')
     for i in reversed(range(nd)):
@@ -200,10 +202,10 @@ def emit_const_kernel(pipe, plan, kname, mode):
         w('  if (%s) { v = %s; pmx_w = true; }' % (' && '.join(pc) or 'true', val))
         w('}')
     w('if (pmx_w) G_%s[idx] = v;' % f.name)
-    L.append('}')
     grid = '((%s + 255) / 256)' % (' * '.join('(long long)%s' % e for e in exts) or '1')
-    return '\n'.join(L) + '\n', KernelInfo(kname, 'const', buffers, grid, 256, 0, plan,
-                                           [g for g in buffers if g is not f], [f])
+    info = KernelInfo(kname, 'const', buffers, grid, 256, 0, plan,
+                      [g for g in buffers if g is not f], [f])
+    return wrap_kernel(kname, params, buffers, [f], 256, 0, L), info
 
 
 class _RedCase(object):
@@ -297,7 +299,7 @@ def _reads_of_reduction(plan):
 
 
 def emit_reduction_kernel(pipe, plan, kname, mode):
-    from .hip_emit import KernelInfo
+    from .hip_emit import KernelInfo, wrap_kernel
     f = plan.func
     params = pipe.sorted_parameters()
     reads = _reads_of_reduction(plan)
@@ -385,12 +387,11 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
         free = plan.free_dims
         nthreads = 256
         w_ = None
-        L.append('extern "C" __global__ __launch_bounds__(%d) void %s(%s) {' % (nthreads, kname, ', '.join(args)))
-        w('__shared__ %s pmx_acc[%d * %d];' % (T, nthreads, plan.nacc))
+        w('%s* pmx_acc = (%s*)pmx_lds;' % (T, T))
         w('%s* acc = pmx_acc + threadIdx.x;' % T)
         ncell = ' * '.join('(long long)%s' % exts[d] for d in od)
         w('const long long pmx_ncell = %s;' % ncell)
-        w('long long cell = (long long)blockIdx.x * %d + threadIdx.x;' % nthreads)
+        w('long long cell = (long long)pmx_tile * %d + threadIdx.x;' % nthreads)
         w('if (cell >= pmx_ncell) return;')
         for d in reversed(od):
             w('const int c%d = (int)(cell %% %s) + (%s); cell /= %s;' % (d, exts[d], lbs[d], exts[d]))
@@ -465,14 +466,13 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
             lin = t if lin is None else '(%s * %s + %s)' % (lin, exts[d], t)
         w('  G_%s[%s] = acc[a * %d];' % (f.name, lin, nthreads))
         w('}')
-        L.append('}')
         grid = '((%s + %d) / %d)' % (ncell, nthreads - 1, nthreads)
-        return '\n'.join(L) + '\n', KernelInfo(kname, 'reduction', buffers, grid, nthreads, 0,
-                                               plan, reads, [f])
+        lds = nthreads * plan.nacc * f.typ.nbytes()
+        info = KernelInfo(kname, 'reduction', buffers, grid, nthreads, lds, plan, reads, [f])
+        return wrap_kernel(kname, params, buffers, [f], nthreads, lds, L), info
 
     # serial fallback: one thread, the reference loop order
-    L.append('extern "C" __global__ __launch_bounds__(64) void %s(%s) {' % (kname, ', '.join(args)))
-    w('if (threadIdx.x != 0 || blockIdx.x != 0) return;')
+    w('if (threadIdx.x != 0) return;')
     lin = None
     ind = 1
     for d in range(nd):
@@ -501,5 +501,5 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
     for j in reversed(range(len(plan.rvars))):
         ind -= 1
         w('}', ind)
-    L.append('}')
-    return '\n'.join(L) + '\n', KernelInfo(kname, 'reduction', buffers, '1', 64, 0, plan, reads, [f])
+    info = KernelInfo(kname, 'reduction', buffers, '1', 64, 0, plan, reads, [f])
+    return wrap_kernel(kname, params, buffers, [f], 64, 0, L), info

@@ -124,6 +124,49 @@ class RegionIR(object):
             vals.append(v)
         return vals
 
+    def evaluate_many(self, env, tiles):
+        """Vectorised ``evaluate``: ``tiles`` maps anchor dims to int64
+        arrays of equal length; returns one array (or scalar) per op."""
+        import numpy as np
+        from .analysis import ceval_np
+        vals = []
+        for op in self.ops:
+            k = op[0]
+            if k == 'c':
+                v = op[1]
+            elif k == 'e':
+                v = eval_bexpr(op[1], env)
+            elif k == 'acc':
+                e2 = dict(env)
+                e2[op[2]] = vals[op[3]]
+                v = ceval_np(op[1], e2)
+            elif k == 'tile':
+                v = np.asarray(tiles[op[1]], dtype=np.int64)
+            elif k == 'add':
+                v = vals[op[1]] + vals[op[2]]
+            elif k == 'sub':
+                v = vals[op[1]] - vals[op[2]]
+            elif k == 'mul':
+                v = vals[op[1]] * op[2]
+            elif k == 'fdiv':
+                v = vals[op[1]] // op[2]
+            elif k == 'min':
+                v = np.minimum(vals[op[1]], vals[op[2]])
+            elif k == 'max':
+                v = np.maximum(vals[op[1]], vals[op[2]])
+            elif k == 'selv':
+                ok = True
+                for p, q in op[1]:
+                    ok = ok & (np.asarray(vals[p]) <= np.asarray(vals[q]))
+                v = np.where(ok, vals[op[2]], vals[op[3]])
+            elif k == 'seleq':
+                v = np.where(np.asarray(vals[op[1]]) == np.asarray(vals[op[2]]),
+                             vals[op[3]], vals[op[4]])
+            else:
+                raise AssertionError(k)
+            vals.append(v)
+        return vals
+
     # printing ----------------------------------------------------------------
     def to_c(self, printer_factory, live=None, prefix='r'):
         """C statements defining ``<prefix><i>`` for every op (or only those

@@ -127,6 +127,40 @@ class BuiltPipeline(object):
             args.append(ctypes.c_void_p(a.ctypes.data))
         check(self.fn_host_rc(*args), 'pipeline_%s' % self.name)
 
+    # -- batches of frames (persistent work-queue launches) ----------------
+    def batch_create(self, frames):
+        """frames: list of dicts like run_dev's ``values`` (device
+        pointers).  Returns an opaque handle for batch_run*."""
+        from .utils import _CTYPE
+        n = len(frames)
+        args = [ctypes.c_int(n)]
+        keep = []
+        for p in self.params:
+            ct = _CTYPE[p.typ.c_type_name()]
+            arr = (ct * n)(*[fr[p.name] for fr in frames])
+            keep.append(arr)
+            args.append(arr)
+        for f in self.inputs + self.outputs:
+            ptrs = []
+            for fr in frames:
+                v = fr[f.name]
+                ptrs.append(v.data_ptr() if hasattr(v, 'data_ptr') else int(v))
+            arr = (ctypes.c_void_p * n)(*ptrs)
+            keep.append(arr)
+            args.append(arr)
+        h = ctypes.c_void_p()
+        args.append(ctypes.byref(h))
+        check(self.lib['pipeline_%s_batch_create' % self.name](*args), 'batch_create')
+        return h
+
+    def batch_run(self, handle, stream=0, mode='persistent'):
+        fn = {'persistent': 'batch_run', 'graph': 'batch_run_graph',
+              'frames': 'batch_run_frames'}[mode]
+        check(self.lib['pipeline_%s_%s' % (self.name, fn)](handle, ctypes.c_void_p(stream)), fn)
+
+    def batch_destroy(self, handle):
+        check(self.lib['pipeline_%s_batch_destroy' % self.name](handle), 'batch_destroy')
+
     def run_dev(self, values, stream=0):
         """Device call: values[name] are device pointers (ints) or objects
         with ``data_ptr()``; ``stream`` is a hipStream_t as int."""
@@ -137,6 +171,11 @@ class BuiltPipeline(object):
             args.append(ctypes.c_void_p(ptr))
         args.append(ctypes.c_void_p(stream))
         check(self.fn_dev(*args), 'pipeline_%s_dev' % self.name)
+
+
+def load(so_path, pipe):
+    """Load an already compiled pipeline library for ``pipe``'s signature."""
+    return BuiltPipeline(pipe, None, so_path)
 
 
 def build(pipe, verbose=False):

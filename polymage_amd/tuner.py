@@ -1,0 +1,360 @@
+"""Autotuning of tile sizes, group sizes and workgroup shape on the GPU.
+
+Two layers:
+
+* The reference interface (sandbox/tuner.py:100-631), kept as is:
+  ``generate(dict)`` builds, emits and compiles one library per
+  (tile size, group size) configuration into ``<dst>/Poly<rand>Mage/``
+  with ``configurations.txt``, and returns ``(dst_dir, n_configs, pipe)``;
+  ``execute(dict)`` loads every ``<app>_polymage_<k>.so``, calls
+  ``pipeline_<name>`` ``_tuner_nruns`` times on the host arrays of
+  ``_tuner_pipe_arg_data`` (or through ``_tuner_custom_executor``) and
+  writes ``tuning_report_<date>.txt`` / ``plot_<date>.dat``.  Build,
+  compile, load and run errors are logged as ``[ERROR]`` and skipped, as
+  in the reference.
+
+* The GPU-native search used for measurement: ``Space`` enumerates
+  candidate schedules, ``compile_all`` compiles them in parallel worker
+  processes (hiprtc + g++; no GPU needed), ``time_candidates`` times each
+  with device-resident inputs and HIP events (median of several runs on a
+  cold buffer set), and ``TimingDB`` appends one JSON record per
+  measurement with the schedule's static features (``costmodel``), which
+  is what the learned cost model trains on.
+"""
+
+import ctypes
+import itertools
+import json
+import os
+import random
+import string
+import sys
+import time
+
+from .compiler import buildPipeline
+from . import runtime
+
+
+# ---------------------------------------------------------------------------
+# reference interface
+# ---------------------------------------------------------------------------
+
+def _print_to(msg, files, end='\n'):
+    for f in files:
+        f.write(str(msg) + end)
+        f.flush()
+
+
+def _line(files):
+    _print_to('-' * 72, files)
+
+
+def generate(arg):
+    app = arg['_tuner_app_name']
+    pipe_name = arg.get('_tuner_pipe_name')
+    live_outs = arg['_tuner_live_outs']
+    constraints = arg.get('_tuner_param_constraints') or []
+    estimates = arg.get('_tuner_param_estimates') or []
+    tiles = arg.get('_tuner_tile_size_configs') or [[16, 16, 16]]
+    groups = arg.get('_tuner_group_size_configs')
+    if not groups:          # the reference's default (it crashes instead, tuner.py:262-267)
+        groups = [1, 2, 3, 200]
+    opts = arg.get('_tuner_opts') or []
+    gpu_opts = arg.get('_tuner_gpu_options') or None
+    dst = arg.get('_tuner_dst_path', '/tmp')
+    debug = arg.get('_tuner_debug_flag', False)
+    dump = [sys.stdout] if debug else []
+
+    def rnd():
+        return ''.join(random.SystemRandom().choice(string.ascii_letters + string.digits)
+                       for _ in range(10))
+    sub = os.path.join(str(dst), 'Poly' + rnd() + 'Mage') + '/'
+    while os.path.exists(sub):
+        sub = os.path.join(str(dst), 'Poly' + rnd() + 'Mage') + '/'
+    os.makedirs(sub)
+    prefix = sub + app + '_polymage_'
+    cfg_path = sub + 'configurations.txt'
+    pipe = None
+    k = 0
+    t_all = time.time()
+    with open(cfg_path, 'a') as cfg:
+        files = dump + [cfg]
+        _line(files)
+        _print_to('App name                : "%s"' % app, files)
+        _print_to('Code Path               : "%s"' % sub, files)
+        _print_to('Live-out Functions      : ' + ' '.join('"%s",' % f.name for f in live_outs), files)
+        _print_to('Tile Sizes              : %s' % (tiles,), files)
+        _print_to('Group Sizes             : %s' % (groups,), files)
+        for ts in tiles:
+            for gs in groups:
+                k += 1
+                _line(files)
+                _print_to('Config     : #%d' % k, files)
+                _print_to('Tile Sizes : %s' % (ts,), files)
+                _print_to('Group Size : %s' % (gs,), files)
+                t1 = time.time()
+                try:
+                    pipe = buildPipeline(live_outs, param_constraints=constraints,
+                                         param_estimates=estimates, tile_sizes=ts,
+                                         group_size=gs, options=opts, pipe_name=pipe_name,
+                                         gpu_options=gpu_opts)
+                    code = pipe.generate_code()
+                except Exception as e:
+                    _print_to('[ERROR] Build fail ... (%s)' % (str(e)[:200],), files)
+                    continue
+                with open(prefix + '%d.hip' % k, 'w') as f:
+                    f.write(str(code))
+                t2 = time.time()
+                _print_to('Code Generation Time   : %sms' % ((t2 - t1) * 1000), files)
+                try:
+                    bp = runtime.build(pipe)
+                    so = prefix + '%d.so' % k
+                    if os.path.lexists(so):
+                        os.remove(so)
+                    os.symlink(bp.so_path, so)
+                except Exception as e:
+                    _print_to('[ERROR] Compilation aborted ... (%s)' % (str(e)[:200],), files)
+                    continue
+                t3 = time.time()
+                _print_to('Compilation Time       : %sms' % ((t3 - t2) * 1000), files)
+                _print_to('Total                  : %sms' % ((t3 - t1) * 1000), files)
+        _line(files)
+        _print_to('Time taken to generate all variants : ', files)
+        _print_to('%sms' % ((time.time() - t_all) * 1000), files)
+        _line(files)
+    return sub, k, pipe
+
+
+def execute(arg):
+    from .utils import get_ordered_cfunc_params, map_cfunc_args
+    custom = arg.get('_tuner_custom_executor')
+    arg_data = arg.get('_tuner_pipe_arg_data')
+    app_data = arg.get('_tuner_app_data')
+    app = arg['_tuner_app_name']
+    pipe = arg['_tuner_pipe']
+    src = arg.get('_tuner_src_path', '/tmp')
+    n = arg.get('_tuner_configs_count', 0)
+    nruns = arg.get('_tuner_nruns', 1)
+    debug = arg.get('_tuner_debug_flag', True)
+    dump = [sys.stdout] if debug else []
+    prefix = os.path.join(str(src), app + '_polymage_')
+    stamp = time.strftime("%d-%m-%Y_%H.%M.%S")
+    report = os.path.join(str(src), 'tuning_report_%s.txt' % stamp)
+    plot = os.path.join(str(src), 'plot_%s.dat' % stamp)
+    fname = 'pipeline_' + pipe.name
+    params = get_ordered_cfunc_params(pipe)
+    args = map_cfunc_args(params, arg_data) if arg_data else None
+    best_k, best_t = 0, None
+    times = {}
+    t0 = time.time()
+    with open(report, 'a') as rep:
+        files = dump + [rep]
+        _line(files)
+        _print_to('App Name              : "%s"' % app, files)
+        _print_to('Total Configurations  :%d' % n, files)
+        _print_to('Number of Tuning Runs :%d' % nruns, files)
+        for k in range(1, n + 1):
+            _line(files)
+            _print_to('Config #%d : ' % k, files, ' ')
+            try:
+                lib = ctypes.cdll.LoadLibrary(prefix + '%d.so' % k)
+                fn = lib[fname]
+            except Exception:
+                _print_to('[ERROR] in loading shared library ...', files)
+                continue
+            local = None
+            err = False
+            for _ in range(nruns):
+                t1 = time.time()
+                try:
+                    if custom is None:
+                        fn(*args)
+                    else:
+                        custom(lib, fn, params, args, arg, app_data)
+                except Exception:
+                    err = True
+                dt = time.time() - t1
+                local = dt if local is None else min(local, dt)
+            if err:
+                _print_to('[ERROR] Execution Aborted ...', files)
+                continue
+            times[k] = local
+            if best_t is None or local < best_t:
+                best_t, best_k = local, k
+            _print_to('%s ms (%s ms)' % (local * 1000, best_t * 1000), files)
+            with open(plot, 'a') as p:
+                p.write('%d %s\n' % (k, local))
+        _line(files)
+        _print_to('Best Config :', files)
+        _print_to('#%d : %s ms' % (best_k, (best_t or 0) * 1000), files)
+        _print_to('Tuning time : %s s' % (time.time() - t0), files)
+    return {'best_config': best_k, 'best_time': best_t, 'times': times, 'report': report}
+
+
+# ---------------------------------------------------------------------------
+# GPU-native search
+# ---------------------------------------------------------------------------
+
+class Candidate(object):
+    def __init__(self, tiles, group_size, threads=256):
+        self.tiles = list(tiles)
+        self.group_size = group_size
+        self.threads = threads
+
+    @property
+    def key(self):
+        return 'T%s_g%d_w%d' % ('x'.join(map(str, self.tiles)), self.group_size, self.threads)
+
+    def __repr__(self):
+        return 'Candidate(%s)' % self.key
+
+
+class Space(object):
+    """Cartesian product of tile sizes, group sizes and threads."""
+
+    def __init__(self, tiles, group_sizes, threads=(256,)):
+        self.candidates = [Candidate(t, g, w) for t, g, w in
+                           itertools.product(tiles, group_sizes, threads)]
+
+    def __iter__(self):
+        return iter(self.candidates)
+
+    def __len__(self):
+        return len(self.candidates)
+
+
+def default_space(app):
+    tiles2 = [(8, 64), (8, 128), (8, 256), (16, 64), (16, 128), (16, 256), (32, 64),
+              (32, 128), (32, 256), (64, 64), (64, 128), (4, 256)]
+    if app == 'unsharp':
+        return Space(tiles2, [1, 5], (128, 256, 512))
+    if app == 'harris':
+        return Space(tiles2, [1, 5, 11], (256,))
+    if app == 'bilateral':
+        return Space([(16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (16, 64), (64, 128),
+                      (8, 64)], [1, 5], (128, 256))
+    if app == 'laplacian':
+        return Space([(16, 32), (16, 64), (32, 32), (32, 64), (8, 64), (32, 128), (16, 128)],
+                     [1, 3, 5, 8], (256,))
+    if app == 'campipe':
+        return Space([(8, 64), (16, 64), (16, 128), (32, 64), (8, 128), (32, 128), (16, 256)],
+                     [3, 5, 12, 40], (256,))
+    raise KeyError(app)
+
+
+def build_candidate(workload_fn, cand, with_data=False):
+    """Build the pipeline of ``workload_fn`` for one candidate."""
+    return workload_fn(tiles=tuple(cand.tiles), group_size=cand.group_size,
+                       gpu_options={'threads': cand.threads}, with_data=with_data)
+
+
+def _compile_one(job):
+    modname, fname, kw, cand_tuple = job
+    import importlib
+    mod = importlib.import_module(modname)
+    fn = getattr(mod, fname)
+    cand = Candidate(*cand_tuple)
+    t0 = time.time()
+    try:
+        w = build_candidate(lambda **k: fn(**dict(kw, **k)), cand)
+        bp_path = runtime.build(w.pipe).so_path
+        from .costmodel import features
+        feats = features(w.pipe)
+        return cand.key, bp_path, feats, time.time() - t0, None
+    except Exception as e:
+        return cand.key, None, None, time.time() - t0, str(e)[:300]
+
+
+def compile_all(workload, candidates, procs=8, **kw):
+    """Compile candidates in parallel; returns {key: (so_path, features,
+    seconds, error)}.  ``workload`` is a function of apps.workloads."""
+    import multiprocessing as mp
+    jobs = [(workload.__module__, workload.__name__, kw,
+             (c.tiles, c.group_size, c.threads)) for c in candidates]
+    out = {}
+    if procs <= 1:
+        res = map(_compile_one, jobs)
+    else:
+        ctx = mp.get_context('fork')
+        pool = ctx.Pool(procs)
+        res = pool.imap_unordered(_compile_one, jobs)
+    for key, so, feats, dt, err in res:
+        out[key] = (so, feats, dt, err)
+    if procs > 1:
+        pool.close()
+        pool.join()
+    return out
+
+
+class TimingDB(object):
+    """Append-only JSON-lines store of measurements."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def add(self, rec):
+        d = os.path.dirname(self.path)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        with open(self.path, 'a') as f:
+            f.write(json.dumps(rec) + '\n')
+
+    def load(self):
+        if not os.path.exists(self.path):
+            return []
+        with open(self.path) as f:
+            return [json.loads(l) for l in f if l.strip()]
+
+
+def time_pipeline(bp, w, runs=7, warmup=2, nsets=3):
+    """Median milliseconds per pipeline call with device-resident data."""
+    import numpy as np
+    import torch
+    dev = torch.device('cuda:0')
+    sets = []
+    for _ in range(nsets):
+        d = {}
+        for f in bp.inputs:
+            a = w.values[f.name]
+            d[f.name] = torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).to(dev)
+        for f in bp.outputs:
+            nb = int(np.prod(w.out_shape)) * f.typ.nbytes()
+            d[f.name] = torch.empty(nb, dtype=torch.uint8, device=dev)
+        sets.append(d)
+    scal = dict((k, v) for k, v in w.values.items() if not hasattr(v, 'shape'))
+    stream = torch.cuda.current_stream().cuda_stream
+    for i in range(warmup):
+        bp.run_dev(dict(scal, **sets[i % nsets]), stream)
+    torch.cuda.synchronize()
+    ts = []
+    for i in range(runs):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        bp.run_dev(dict(scal, **sets[i % nsets]), stream)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def time_candidates(workload, compiled, runs=7, db=None, app=None, **kw):
+    """Time compiled candidates on the GPU; returns {key: ms}."""
+    w = workload(with_data=True, **kw)
+    out = {}
+    for key, (so, feats, dt, err) in compiled.items():
+        if so is None:
+            continue
+        bp = runtime.load(so, w.pipe)
+        try:
+            ms = time_pipeline(bp, w, runs=runs)
+        except Exception as e:
+            if db is not None:
+                db.add({'app': app, 'key': key, 'error': str(e)[:200]})
+            continue
+        out[key] = ms
+        if db is not None:
+            db.add({'app': app, 'key': key, 'ms': ms, 'mpix': w.mpix, 'features': feats,
+                    'compile_s': dt})
+    return out

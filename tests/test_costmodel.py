@@ -1,0 +1,74 @@
+"""Cost model: the oracle MLP is pinned by finite differences (CPU); the
+HIP forward/backward/Adam kernels match it (GPU); features are computed
+for real schedules (CPU)."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+
+
+def test_oracle_mlp_gradient_finite_difference():
+    rng = np.random.default_rng(0)
+    dims = [5, 7, 3, 1]
+    n = sum(dims[l] * dims[l + 1] + dims[l + 1] for l in range(3))
+    p = rng.normal(0, 0.5, n)
+    x = rng.normal(0, 1, (9, 5))
+    t = rng.normal(0, 1, (9, 1))
+    loss, g = mlp_ref.loss_and_grad(p, dims, x, t)
+    eps = 1e-6
+    for i in rng.choice(n, 25, replace=False):
+        q = p.copy()
+        q[i] += eps
+        lp, _ = mlp_ref.loss_and_grad(q, dims, x, t)
+        q[i] -= 2 * eps
+        lm, _ = mlp_ref.loss_and_grad(q, dims, x, t)
+        assert abs((lp - lm) / (2 * eps) - g[i]) < 1e-6 * max(1.0, abs(g[i]))
+
+
+def test_features_of_real_schedules():
+    from polymage_amd.apps import workloads
+    from polymage_amd.costmodel import features, vector, FEATURES
+    f1 = features(workloads.unsharp(tiles=(32, 256), with_data=False).pipe)
+    f2 = features(workloads.unsharp(tiles=(8, 64), with_data=False).pipe)
+    assert vector(f1).shape == (len(FEATURES),)
+    assert f1['n_kernels'] == 1 and f1['n_stages'] == 4
+    # smaller tiles -> more halo recompute and more tiles
+    assert f2['overlap_ratio'] > f1['overlap_ratio']
+    assert f2['log_total_tiles'] > f1['log_total_tiles']
+
+
+@pytest.mark.gpu
+def test_hip_mlp_matches_oracle(gpu):
+    import torch
+    from polymage_amd.costmodel import MLP
+    rng = np.random.default_rng(1)
+    m = MLP(dims=[6, 16, 8, 1], seed=3)
+    x = rng.normal(0, 1, (37, 6)).astype(np.float32)
+    t = rng.normal(0, 1, (37, 1)).astype(np.float32)
+    p = m.params.cpu().numpy().astype(np.float64)
+    xs = torch.from_numpy(x).to(m.device)
+    ts = torch.from_numpy(t).to(m.device)
+    out, acts = m.forward_raw(xs)
+    ref = mlp_ref.forward(p, m.dims, x.astype(np.float64))[-1]
+    assert np.allclose(out.cpu().numpy(), ref, rtol=1e-4, atol=1e-5)
+    grads, loss = m.backward_raw(acts, ts)
+    rloss, rg = mlp_ref.loss_and_grad(p, m.dims, x.astype(np.float64), t.astype(np.float64))
+    assert abs(float(loss.item()) - rloss) < 1e-4 * max(1.0, rloss)
+    assert np.allclose(grads.cpu().numpy(), rg, rtol=1e-3, atol=1e-5)
+    # one Adam step
+    mm = np.zeros_like(p)
+    vv = np.zeros_like(p)
+    pr, _, _ = mlp_ref.adam(p, grads.cpu().numpy().astype(np.float64), mm, vv, 1, 1e-2)
+    m.adam(grads, 1e-2)
+    assert np.allclose(m.params.cpu().numpy(), pr, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_hip_mlp_training_reduces_loss(gpu):
+    from polymage_amd.costmodel import MLP
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, (256, 8))
+    y = np.sin(X[:, 0]) + 0.5 * X[:, 1] ** 2 - X[:, 2] * X[:, 3]
+    m = MLP(dims=[8, 32, 32, 1], seed=0)
+    losses = m.fit(X, y, epochs=600, lr=3e-3)
+    assert losses[-1] < 0.2 * losses[0]

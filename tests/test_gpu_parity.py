@@ -111,3 +111,128 @@ def test_bilateral_full_size(gpu):
     print("case bilateral 2560x1536", flush=True)
     g, o = _bilateral(2560, 1536, [32, 64], seed=1003)
     assert bits_equal(g, o)
+
+
+def _laplacian(Rin, Cin, tiles, K=8, J=4, io='u16', mode='double', seed=9, group_size=5):
+    from polymage_amd.apps.local_laplacian import local_laplacian
+    pd = {}
+    out = local_laplacian(pd, K=K, J=J, io=io, lut_hi_inclusive=(io == 'u16'))
+    lo, hi = pd['pad']
+    R, C = Rin + lo + hi, Cin + lo + hi
+    pipe = build(out, pd, {'R': R, 'C': C}, 'laplacian', tiles, group_size=group_size,
+                 literal_mode=mode)
+    g_ = np.random.default_rng(seed)
+    if io == 'u16':
+        img = np.ascontiguousarray((inputs.smooth_field(g_, Rin, Cin, 3) * 65535).round()
+                                   .astype(np.uint16).transpose(1, 2, 0))
+        dt = np.uint16
+    else:
+        img = g_.integers(0, 256, (Rin, Cin, 3)).astype(np.uint8)
+        dt = np.uint8
+    alpha = 1.0 / (K - 1)
+    vals = {'R': R, 'C': C, 'alpha': alpha, 'beta': 1.0, 'img_colour': img}
+    g = np.zeros((Rin, Cin, 3), dt)
+    o = np.zeros((Rin, Cin, 3), dt)
+    orc = cemit.build([out], 'lap_oracle_%d_%d_%s_%s' % (K, J, io, mode), literal_mode=mode)
+    gpu_build(pipe).run_host(dict(vals, laplacian=g))
+    orc(dict(vals, laplacian=o))
+    return g, o
+
+
+@pytest.mark.parametrize("Rin,Cin,tiles", [(96, 80, [16, 32]), (200, 136, [32, 64])])
+def test_laplacian_u16_bitexact(gpu, Rin, Cin, tiles):
+    print("case laplacian u16", Rin, Cin, tiles, flush=True)
+    g, o = _laplacian(Rin, Cin, tiles)
+    assert np.array_equal(g, o)
+
+
+def test_laplacian_video_demo_form(gpu):
+    print("case laplacian u8 K4 J5", flush=True)
+    g, o = _laplacian(160, 128, [32, 64], K=4, J=5, io='u8', mode='float12')
+    assert np.array_equal(g, o)
+
+
+def test_laplacian_full_size(gpu):
+    print("case laplacian 2560x1536", flush=True)
+    g, o = _laplacian(2560, 1536, [32, 64], seed=1004)
+    assert np.array_equal(g, o)
+
+
+def _campipe_frames(n, seed, variant='batch', sizes=None):
+    g = np.random.default_rng(seed)
+    frames = []
+    for i in range(n):
+        if sizes:
+            R, C = sizes[i]
+        else:
+            R, C = 32 * int(g.integers(12, 20)), 32 * int(g.integers(16, 24))
+        dt = np.uint16 if variant == 'batch' else np.int16
+        frames.append(g.integers(0, 1024, (R, C)).astype(dt))
+    return frames
+
+
+def _campipe_vals(img):
+    from polymage_amd.apps.campipe import MATRIX_3200, MATRIX_7000
+    R, C = img.shape
+    return {'R': R, 'C': C, 'colour_temp': 3700.0, 'gamma': 2.2, 'contrast': 50.0, 'img': img,
+            'matrix_3200': np.array(MATRIX_3200, np.float32),
+            'matrix_7000': np.array(MATRIX_7000, np.float32)}
+
+
+@pytest.mark.parametrize("variant,gs", [('reference', 5), ('batch', 5), ('batch', 40)])
+def test_campipe_bitexact(gpu, variant, gs):
+    from polymage_amd.apps.campipe import camera_pipe
+    print("case campipe", variant, gs, flush=True)
+    pd = {}
+    out = camera_pipe(pd, variant=variant)
+    img = _campipe_frames(1, 21, variant, sizes=[(416, 608)])[0]
+    R, C = img.shape
+    pipe = build(out, pd, {'R': R, 'C': C}, 'campipe', [16, 64], group_size=gs)
+    vals = _campipe_vals(img)
+    g = np.zeros((3, R - 48, C - 64), np.uint8)
+    o = np.zeros_like(g)
+    orc = cemit.build([out], 'cam_oracle_' + variant)
+    gpu_build(pipe).run_host(dict(vals, process=g))
+    orc(dict(vals, process=o))
+    assert np.array_equal(g, o)
+
+
+def test_campipe_batch_persistent_and_graph(gpu):
+    """Frames of different sizes in one persistent launch per kernel (and
+    the same captured in a hipGraph, replayed twice) give the oracle's
+    output for every frame."""
+    import torch
+    from polymage_amd.apps.campipe import camera_pipe
+    print("case campipe batch", flush=True)
+    pd = {}
+    out = camera_pipe(pd, variant='batch')
+    frames = _campipe_frames(6, 22)
+    Rm = max(f.shape[0] for f in frames)
+    Cm = max(f.shape[1] for f in frames)
+    pipe = build(out, pd, {'R': Rm, 'C': Cm}, 'campipe', [16, 64], group_size=40)
+    bp = gpu_build(pipe)
+    orc = cemit.build([out], 'cam_oracle_batch')
+    dev = torch.device('cuda:0')
+    recs, outs, refs = [], [], []
+    m32 = torch.from_numpy(_campipe_vals(frames[0])['matrix_3200']).to(dev)
+    m70 = torch.from_numpy(_campipe_vals(frames[0])['matrix_7000']).to(dev)
+    for img in frames:
+        R, C = img.shape
+        vals = _campipe_vals(img)
+        o = np.zeros((3, R - 48, C - 64), np.uint8)
+        orc(dict(vals, process=o))
+        refs.append(o)
+        d_img = torch.from_numpy(img.view(np.int16)).to(dev)
+        d_out = torch.zeros((3, R - 48, C - 64), dtype=torch.uint8, device=dev)
+        outs.append(d_out)
+        recs.append(dict(vals, img=d_img, process=d_out, matrix_3200=m32, matrix_7000=m70))
+    h = bp.batch_create(recs)
+    stream = torch.cuda.current_stream().cuda_stream
+    for mode in ('persistent', 'graph', 'graph', 'frames'):
+        for d in outs:
+            d.zero_()
+        bp.batch_run(h, stream, mode)
+        torch.cuda.synchronize()
+        for d, r in zip(outs, refs):
+            assert np.array_equal(d.cpu().numpy(), r), mode
+    bp.batch_destroy(h)

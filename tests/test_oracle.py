@@ -129,3 +129,49 @@ def test_oracle_vs_video_demo_laplacian():
     assert np.array_equal(o[:Rin - 40], ref[:Rin - 40])
     rows = np.unique(np.nonzero(o != ref)[0])
     assert rows.size == 0 or rows.min() >= Rin - 40
+
+
+def _eval64_check(out, vals, outname, oracle_name, shape, dtype=np.float32, tol=1e-6, mode='double'):
+    from oracle import eval64
+    o = np.zeros(shape, dtype)
+    cemit.build([out], oracle_name, literal_mode=mode)(dict(vals, **{outname: o}))
+    r = eval64.evaluate([out], vals)[outname]
+    st = eval64.error_statement(o, r)
+    assert st['max_rel_normwise'] < tol, st
+    return o, r
+
+
+def test_oracle_vs_eval64_float_pipelines():
+    """The C oracle against the independent float64 evaluator: equal to
+    float32 precision (normwise relative error)."""
+    from polymage_amd.apps import inputs
+    from polymage_amd.apps.bilateral import bilateral_grid_full
+    R = C = 96
+    img = np.random.default_rng(1).random((R + 2, C + 2)).astype(np.float32)
+    _eval64_check(harris_pipe({}), {'R': R, 'C': C, 'img': img}, 'harris', 'harris_oracle_double',
+                  (R + 2, C + 2))
+    img = inputs.smooth_field(np.random.default_rng(2), R + 4, C + 4, 3)
+    _eval64_check(unsharp_mask_full({}), {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0, 'img': img},
+                  'mask', 'unsharp_oracle_unsharp_mask_full', (3, R, C))
+    img = inputs.smooth_field(np.random.default_rng(3), 128, 96, 1)[0]
+    _eval64_check(bilateral_grid_full({}), {'rows': 128, 'cols': 96, 'input': img}, 'filtered',
+                  'bilateral_oracle_full', (1, 128, 96))
+
+
+def test_oracle_vs_eval64_integer_pipeline():
+    """Camera pipeline (integer stages): the float64 evaluator reproduces
+    the oracle exactly except where a float32-vs-float64 tone-curve entry
+    straddles an integer boundary."""
+    from oracle import eval64
+    from polymage_amd.apps.campipe import camera_pipe, MATRIX_3200, MATRIX_7000
+    out = camera_pipe({}, variant='batch')
+    R, C = 160, 224
+    img = np.random.default_rng(4).integers(0, 1024, (R, C)).astype(np.uint16)
+    vals = {'R': R, 'C': C, 'colour_temp': 3700.0, 'gamma': 2.2, 'contrast': 50.0, 'img': img,
+            'matrix_3200': np.array(MATRIX_3200, np.float32),
+            'matrix_7000': np.array(MATRIX_7000, np.float32)}
+    o = np.zeros((3, R - 48, C - 64), np.uint8)
+    cemit.build([out], 'cam_oracle_batch')(dict(vals, process=o))
+    r = eval64.evaluate([out], vals)['process']
+    diff = np.abs(o.astype(np.int64) - r)
+    assert diff.max() <= 2 and np.mean(diff > 0) < 0.01
