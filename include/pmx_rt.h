@@ -61,6 +61,10 @@ int pmx_launch(void* function, unsigned gx, unsigned gy, unsigned gz,
                void* stream, void** args);
 
 /* ---- streams, events, graphs --------------------------------------------- */
+/* Streams are created at the default priority (non-blocking w.r.t. the
+ * null stream); graph capture needs a stream other than the null stream. */
+int pmx_stream_create(void** stream);
+int pmx_stream_destroy(void* stream);
 int pmx_stream_sync(void* stream);
 int pmx_event_create(void** ev);
 int pmx_event_destroy(void* ev);

@@ -174,6 +174,17 @@ int pmx_launch(void* function, unsigned gx, unsigned gy, unsigned gz,
                    "hipModuleLaunchKernel");
 }
 
+int pmx_stream_create(void** stream) {
+  hipStream_t s;
+  int rc = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  if (rc) return rc;
+  *stream = (void*)s;
+  return 0;
+}
+int pmx_stream_destroy(void* stream) {
+  return hip_check(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+}
+
 int pmx_stream_sync(void* stream) {
   return hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
 }

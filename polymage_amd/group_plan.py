@@ -400,37 +400,42 @@ class GroupPlan(object):
             s.last_read = max(reads) if reads else s.phase
 
     def _storage(self):
-        # storage classes: (element type, static extents); reuse a freed
-        # buffer of the same class (storage_mapping.py:387-459)
+        """LDS layout with reuse.  Storage class = (element type, rank), as
+        the reference's optimize_storage keys classes on type and
+        dimensionality and sizes them by the largest member
+        (storage_mapping.py:152-299); a buffer whose last reader ran in an
+        earlier phase is reused by a later stage of its class when large
+        enough (best fit), otherwise new space is appended."""
         free = {}
-        buffers = []     # (offset, bytes, class)
         offset = 0
         order = sorted([s for s in self.stages if s.needs_lds],
                        key=lambda s: (s.phase, s.index))
-        live = []        # (last_read, stage)
+        live = []
         for s in order:
-            # release buffers whose last read is before this phase
             still = []
             for (lr, t) in live:
                 if lr < s.phase:
-                    free.setdefault(t.lds_class, []).append(t.lds_offset)
+                    free.setdefault(t.lds_class, []).append((t.lds_offset, t.lds_size))
                 else:
                     still.append((lr, t))
             live = still
             nbytes = s.func.typ.nbytes()
             for e in s.static_ext:
                 nbytes *= e
-            cls = (s.func.typ.c_name(), tuple(s.static_ext))
+            nbytes = (nbytes + 15) // 16 * 16
+            cls = (s.func.typ.c_name(), s.nd)
             s.lds_class = cls
-            if free.get(cls):
-                s.lds_offset = free[cls].pop(0)
+            cands = sorted([fs for fs in free.get(cls, []) if fs[1] >= nbytes], key=lambda fs: fs[1])
+            if cands:
+                slot = cands[0]
+                free[cls].remove(slot)
+                s.lds_offset, s.lds_size = slot
             else:
-                align = 16
-                offset = (offset + align - 1) // align * align
                 s.lds_offset = offset
+                s.lds_size = nbytes
                 offset += nbytes
             live.append((s.last_read, s))
-        self.lds_bytes = (offset + 15) // 16 * 16
+        self.lds_bytes = offset
         if self.lds_bytes > LDS_LIMIT_BYTES:
             raise PlanError("group %d needs %d bytes of LDS (> %d); use smaller tiles"
                             % (self.gindex, self.lds_bytes, LDS_LIMIT_BYTES))

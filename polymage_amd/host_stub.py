@@ -355,9 +355,14 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     w('  pmx_batch_%s* B = (pmx_batch_%s*)handle;' % (name, name))
     w('  int rc = 0;')
     w('  if (!B->graph) {')
-    w('    rc = pmx_graph_begin(stream); if (rc) return rc;')
-    w('    int rc2 = pipeline_%s_batch_run(handle, stream);' % name)
-    w('    rc = pmx_graph_end(stream, &B->graph); if (rc2) return rc2; if (rc) return rc;')
+    w('    // capture on a private stream (the null stream cannot be captured);')
+    w('    // the instantiated graph is then launched on the caller\'s stream')
+    w('    void* cs = nullptr; rc = pmx_stream_create(&cs); if (rc) return rc;')
+    w('    rc = pmx_graph_begin(cs); if (rc) { pmx_stream_destroy(cs); return rc; }')
+    w('    int rc2 = pipeline_%s_batch_run(handle, cs);' % name)
+    w('    rc = pmx_graph_end(cs, &B->graph);')
+    w('    pmx_stream_destroy(cs);')
+    w('    if (rc2) return rc2; if (rc) return rc;')
     w('  }')
     w('  return pmx_graph_launch(B->graph, stream);')
     w('}')

@@ -40,3 +40,10 @@ def gpu_build(pipe):
     if os.environ.get('PMX_PREBUILD'):
         pytest.skip("prebuild only")
     return bp
+
+
+def skip_if_prebuild():
+    import os
+    import pytest
+    if os.environ.get('PMX_PREBUILD'):
+        pytest.skip("prebuild only")

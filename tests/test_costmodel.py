@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import mlp_ref
+from helpers import skip_if_prebuild
 
 
 def test_oracle_mlp_gradient_finite_difference():
@@ -39,6 +40,7 @@ def test_features_of_real_schedules():
 
 @pytest.mark.gpu
 def test_hip_mlp_matches_oracle(gpu):
+    skip_if_prebuild()
     import torch
     from polymage_amd.costmodel import MLP
     rng = np.random.default_rng(1)
@@ -65,6 +67,7 @@ def test_hip_mlp_matches_oracle(gpu):
 
 @pytest.mark.gpu
 def test_hip_mlp_training_reduces_loss(gpu):
+    skip_if_prebuild()
     from polymage_amd.costmodel import MLP
     rng = np.random.default_rng(2)
     X = rng.normal(0, 1, (256, 8))
