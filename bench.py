@@ -35,9 +35,27 @@ def parse():
     ap.add_argument('--threads', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
-    ap.add_argument('--traffic', default=None,
-                    help='JSON file with measured FETCH/WRITE KiB per dispatch')
+    ap.add_argument('--traffic', default=os.path.join(ROOT, 'profiles', 'traffic.json'),
+                    help='JSON file with the HBM bytes per dispatch measured by rocprofv3 PMC runs')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                    help='instead of the pipeline, run a plain device copy of MIB MiB '
+                         '(calibrates the FETCH_SIZE / WRITE_SIZE counters)')
     return ap.parse_args()
+
+
+def copy_probe(mib, steps):
+    """A plain device-to-device copy of known size, for scaling the PMC
+    byte counters: FETCH_SIZE and WRITE_SIZE of one dispatch should read
+    ``mib`` MiB each."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device='cuda:0')
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_probe_mib': mib, 'dispatches': steps}))
 
 
 def cpu_baseline(w, budget_s):
@@ -72,6 +90,9 @@ def main():
         print('bench.py: this build measures one GPU only; --gpus %d is not supported '
               '(multi-GPU sharding is described in DESIGN.md)' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe, args.steps)
+        return
     import numpy as np
     import torch
     from polymage_amd import runtime
@@ -130,7 +151,10 @@ def main():
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
-            traffic = json.load(f).get(w.name)
+            rec = json.load(f).get(w.name)
+        # a record applies only to the kernel source it was measured on
+        if rec and rec.get('code_key') == bp.code.key:
+            traffic = rec.get('bytes_per_dispatch')
 
     result = {
         'metric': 'MPix/s per pipeline, 1 MI355X: learned vs exhaustive schedule; % HBM roofline',

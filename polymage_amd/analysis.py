@@ -254,6 +254,48 @@ def classify_access(arg, consumer_vars):
                   identity=False, increasing=(slope > 0))
 
 
+_NON_SHIFT_OPS = ('/', '%', '<<', '>>', '&', '|', '^')
+
+
+def shift_access(arg, consumer_vars):
+    """``(k, off)`` when ``arg`` is exactly ``consumer_vars[k] + off`` with
+    ``off`` built from parameters and integer constants by + - * only (no
+    truncating operator, so the identity holds in C integer arithmetic);
+    ``('const', arg)`` for a parameter/constant argument; else None.
+    ``off`` is an int when it is a plain constant."""
+    from .expression import substitute_vars, is_constant_expr, get_constant_from_expr
+    arg = Value.numericToValue(arg)
+    if arg.has(Reference) or arg.has(Select) or arg.has(Cast) or arg.has(InbuiltFunction):
+        return None
+    stack = [arg]
+    while stack:
+        e = stack.pop()
+        if isinstance(e, AbstractBinaryOpNode) and e.op in _NON_SHIFT_OPS:
+            return None
+        if hasattr(e, 'children'):
+            stack.extend(e.children())
+    allv = arg.collect(Variable) if arg.has(Variable) else []
+    used = [v for v in consumer_vars if v in allv]
+    others = [v for v in allv if v not in consumer_vars and not isinstance(v, Parameter)]
+    if others or len(used) > 1:
+        return None
+    if not used:
+        return ('const', arg)
+    v = used[0]
+    if not isAffine(arg, include_div=False):
+        return None
+    coeff = get_affine_var_and_param_coeff(arg)
+    if coeff.get(v) != 1:
+        return None
+    off = substitute_vars(arg, {v: Value(0, Int)})
+    off = Value.numericToValue(off)
+    if is_constant_expr(off):
+        c = get_constant_from_expr(off)
+        if c == int(c):
+            off = int(c)
+    return (consumer_vars.index(v), off)
+
+
 def interval_image(acc, lo, hi, env):
     """Exact [min, max] of a monotone access over consumer values [lo, hi]
     (other variables bound in env)."""

@@ -31,7 +31,7 @@ import itertools
 import math
 
 from .analysis import (PlanError, classify_access, case_parts, ceval_int,
-                       eval_bexpr)
+                       eval_bexpr, shift_access)
 from .constructs import Reference, Reduction
 from .region_ir import RegionIR, BIG
 
@@ -91,6 +91,7 @@ class GroupPlan(object):
         self._regions()
         self._sizes()
         self._phases()
+        self._blocks()
         self._storage()
 
     # -- structure -------------------------------------------------------------
@@ -255,6 +256,7 @@ class GroupPlan(object):
             hi_s = ir.sel_eq(t, last, big, hi_s)
             return lo_s, hi_s
 
+        self._requests = []     # (ref, dim, producer, valid boxes, lo id, hi id)
         for s in reversed(self.stages):
             lo = [big] * s.nd
             hi = [nbig] * s.nd
@@ -296,6 +298,7 @@ class GroupPlan(object):
                         al, ah = dom_ids(s, i)
                     lo[i] = ir.sel_valid(valid, ir.min(lo[i], al), lo[i])
                     hi[i] = ir.sel_valid(valid, ir.max(hi[i], ah), hi[i])
+                    self._requests.append((r, i, s, valid, al, ah))
             reg = []
             for i in range(s.nd):
                 dlo, dhi = dom_ids(s, i)
@@ -341,6 +344,22 @@ class GroupPlan(object):
         else:
             tiles = {}
         vals = self.ir.evaluate_many(self.env, tiles)
+        # references that can ask for producer points outside the producer's
+        # computed region (its domain or Case hull clipped the request: the
+        # program reads points it never defines) get clamped reads
+        self.clamp_refs = set()
+        for (r, i, p, valid, al, ah) in self._requests:
+            ok = None
+            for (l, h) in valid:
+                c = np.asarray(vals[l]) <= np.asarray(vals[h])
+                ok = c if ok is None else (ok & c)
+            rl, rh = p.region[i]
+            bad = (np.asarray(vals[al]) < np.asarray(vals[rl])) | \
+                  (np.asarray(vals[ah]) > np.asarray(vals[rh]))
+            if ok is not None:
+                bad = bad & ok
+            if np.any(bad):
+                self.clamp_refs.add((id(r), i))
         ext = {}
         for s in self.stages:
             ext[s] = []
@@ -399,6 +418,91 @@ class GroupPlan(object):
             s.needs_lds = bool(reads)
             s.last_read = max(reads) if reads else s.phase
 
+    # -- per-thread register blocks ------------------------------------------------
+    def closure_refs(self, s):
+        """[(stage whose variables the reference's arguments use, Reference)]
+        over the parts of s and of every stage inlined into it."""
+        out = []
+        seen = set()
+
+        def visit(t):
+            for part in t.parts:
+                for r in self._part_refs(part):
+                    out.append((t, r))
+                    if self.in_group(r.objectRef):
+                        p = self.stage_of(r.objectRef)
+                        if p.inline and p not in seen:
+                            seen.add(p)
+                            visit(p)
+        visit(s)
+        return out
+
+    def _blocks(self):
+        """Choose each materialised stage's per-thread block: U consecutive
+        points along its second-innermost dimension times V along the
+        innermost one.  A thread computes the U*V points of an item in
+        straight-line code, so loads (and their conversions) shared by
+        neighbouring points are issued once and consecutive loads merge
+        into 128-bit accesses.  V = 4 only for stages that read no LDS
+        (lane-strided LDS accesses would conflict on banks); U and V
+        minimise an estimate of thread time: sweep iterations times points
+        per item times per-point cost, where the per-point load count
+        shrinks with the reuse a block gives to the stage's stencil reach."""
+        threads = int(self.opts.get('threads', 256))
+        forced = self.opts.get('block')
+        for s in self.stages:
+            s.block = (1, 1)
+            if s.inline or s.nd == 0:
+                continue
+            ext = s.static_ext
+            ext_c = ext[-1]
+            ext_r = ext[-2] if s.nd >= 2 else 1
+            outer = 1
+            for e in ext[:-2]:
+                outer *= e
+            reads_lds = False
+            rv = rh = 0
+            for (t, r) in self.closure_refs(s):
+                if self.in_group(r.objectRef) and not self.stage_of(r.objectRef).inline:
+                    reads_lds = True
+                for arg in r.arguments:
+                    sa = shift_access(arg, t.func.variables)
+                    if sa is None or sa[0] == 'const' or not isinstance(sa[1], int):
+                        continue
+                    if sa[0] == s.nd - 1:
+                        rh = max(rh, abs(sa[1]))
+                    elif s.nd >= 2 and sa[0] == s.nd - 2:
+                        rv = max(rv, abs(sa[1]))
+            if forced is not None:
+                U, V = forced
+                U = U if s.nd >= 2 else 1
+                V = V if (not reads_lds and ext_c >= 4 * V) else 1
+                s.block = (max(1, min(U, ext_r)), V)
+                continue
+            Vs = [4, 1] if (not reads_lds and ext_c >= 16) else [1]
+            best = None
+            for V in Vs:
+                for U in (1, 2, 4, 8):
+                    if U > 1 and (s.nd < 2 or ext_r < 2 * U) or U * V > 16:
+                        continue
+                    items = outer * (-(-ext_r // U)) * (-(-ext_c // V))
+                    iters = -(-items // threads)
+                    taps = (U + 2.0 * rv) / U * (V + 2.0 * rh) / V
+                    cost = iters * (U * V * (8.0 + 2.0 * taps) + 6.0)
+                    key = (cost, -U * V)
+                    if best is None or key < best[0]:
+                        best = (key, (U, V))
+            s.block = best[1]
+
+    def lds_ext(self, s):
+        """LDS layout extents of a stage: the static box, with the innermost
+        extent padded to a multiple of 4 for 4-wide blocks (aligned
+        128-bit LDS stores)."""
+        ext = list(s.static_ext)
+        if s.block[1] == 4:
+            ext[-1] = (ext[-1] + 3) // 4 * 4
+        return ext
+
     def _storage(self):
         """LDS layout with reuse.  Storage class = (element type, rank), as
         the reference's optimize_storage keys classes on type and
@@ -420,7 +524,7 @@ class GroupPlan(object):
                     still.append((lr, t))
             live = still
             nbytes = s.func.typ.nbytes()
-            for e in s.static_ext:
+            for e in self.lds_ext(s):
                 nbytes *= e
             nbytes = (nbytes + 15) // 16 * 16
             cls = (s.func.typ.c_name(), s.nd)

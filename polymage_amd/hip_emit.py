@@ -22,7 +22,7 @@ reference, and ``pipeline_<name>_dev(..., stream)`` on device pointers.
 
 import hashlib
 
-from .analysis import PlanError, classify_access, eval_bexpr, unconditional_refs
+from .analysis import PlanError, classify_access, eval_bexpr, unconditional_refs, shift_access
 from .constructs import Reference, Image, Case, Reduction
 from .cprint import Printer, DEVICE_PRELUDE, c_type
 from .group_plan import GroupPlan
@@ -152,6 +152,9 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         for d in range(nd):
             w('const %s E_%s_%d = %s;' % (index_t, f.name, d, _extent_c(f, d, base_pr)))
             w('const int LB_%s_%d = %s;' % (f.name, d, base_pr.expr(f.domain[d].lowerBound)))
+        for d in range(nd - 1):
+            w('const %s S_%s_%d = %s;' % (index_t, f.name, d, ' * '.join(
+                'E_%s_%d' % (f.name, j) for j in range(d + 1, nd))))
 
     def gindex(f, idx_strs):
         e = None
@@ -164,30 +167,46 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         e = None
         for d, q in enumerate(idx_strs):
             term = '(%s - %s)' % (q, R(p.region[d][0]))
-            e = term if e is None else '(%s * %d + %s)' % (e, p.static_ext[d], term)
+            e = term if e is None else '(%s * %d + %s)' % (e, plan.lds_ext(p)[d], term)
         return e if e is not None else '0'
 
     # references that only run inside a Select branch: their global reads
     # are clamped to the producer's domain (the run-time bounds check in
     # the host stub covers unconditional references only)
+    # A branch reference naming the same point as an unconditional
+    # reference of the same part reads an address that is read anyway, so
+    # it needs no clamp either.
     uncond = set()
     for s in stages:
         for part in s.parts:
+            keys = set()
             for r in unconditional_refs(part.expr):
                 uncond.add(id(r))
+                keys.add((id(r.objectRef), tuple(str(a) for a in r.arguments)))
             for c in part.pred:
                 for r in c.collect(Reference):
                     uncond.add(id(r))
+            for r in part.expr.collect(Reference):
+                if (id(r.objectRef), tuple(str(a) for a in r.arguments)) in keys:
+                    uncond.add(id(r))
 
-    def make_ref_cb(cur_stage):
+    def make_ref_cb(cur_stage, item=None):
         def cb(ref, pr, pro):
             f = ref.objectRef
             cvars = cur_stage.func.variables
+            clamped = any((id(ref), d) in plan.clamp_refs for d in range(len(ref.arguments)))
+            if item is not None and not clamped and \
+                    not (plan.in_group(f) and plan.stage_of(f).inline) and \
+                    (plan.in_group(f) or id(ref) in uncond):
+                sh = [shift_access(arg, cvars) for arg in ref.arguments]
+                if all(x is not None for x in sh):
+                    return item.ref(f, sh, pr)
             idx = []
             for d, arg in enumerate(ref.arguments):
                 acc = classify_access(arg, cvars)
                 q = pr.expr(arg, pro)
-                if acc.kind == '*' or (id(ref) not in uncond and not plan.in_group(f)):
+                if acc.kind == '*' or (id(ref), d) in plan.clamp_refs or \
+                        (id(ref) not in uncond and not plan.in_group(f)):
                     if plan.in_group(f):
                         p = plan.stage_of(f)
                         q = 'pmx_clampi(%s, %s, %s)' % (q, R(p.region[d][0]), R(p.region[d][1]))
@@ -202,6 +221,94 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 return 'L_%s[%s]' % (p.name, lds_index(p, idx))
             return '%s[%s]' % (_gname(f), gindex(f, idx))
         return cb
+
+    class Item(object):
+        """Addressing of one per-thread block: point = item base ``pb`` +
+        offset ``d`` per stage dimension.  A reference whose arguments are
+        consumer variables plus parameter/constant offsets becomes a per-item
+        base (pointer or LDS index, computed once) plus an offset that is a
+        compile-time constant or uniform, so neighbouring points share the
+        address arithmetic and their loads."""
+
+        def __init__(self, pb, d):
+            self.pb = pb
+            self.d = d
+            self.decls = []
+            self.names = {}
+
+        def _base(self, key, make):
+            if key not in self.names:
+                nm = 'ib%d' % len(self.names)
+                self.names[key] = nm
+                self.decls.append(make(nm))
+            return self.names[key]
+
+        def _off(self, parts, pr):
+            """Sum of (offset, stride) pairs as C text; integer offsets with
+            integer strides fold into one constant."""
+            terms = []
+            const = 0
+            for (o, stride) in parts:
+                if isinstance(o, int):
+                    if o == 0:
+                        continue
+                    if isinstance(stride, int):
+                        const += o * stride
+                    else:
+                        terms.append('(%d * %s)' % (o, stride))
+                elif stride == 1:
+                    terms.append('(%s)' % o)
+                else:
+                    terms.append('((%s) * %s)' % (o, stride))
+            if const or not terms:
+                terms.append('%d' % const)
+            return ' + '.join(terms)
+
+        def _dsum(self, k, off, pr):
+            d = self.d[k]
+            if isinstance(off, int):
+                if isinstance(d, int):
+                    return d + off
+                return '%s + %d' % (d, off) if off else d
+            o = pr.expr(off)
+            return '%s + %s' % (d, o) if not (isinstance(d, int) and d == 0) else o
+
+        def ref(self, f, sh, pr):
+            kinds = tuple(x[0] for x in sh)
+            parts = []
+            if plan.in_group(f):
+                p = plan.stage_of(f)
+                ext = plan.lds_ext(p)
+                W = [_prod(ext[j + 1:]) for j in range(len(ext))]
+
+                def mk(nm):
+                    terms = []
+                    for j, (k, off) in enumerate(sh):
+                        pbk = self.pb[k] if k != 'const' else '0'
+                        terms.append('(%s - %s) * %d' % (pbk, R(p.region[j][0]), W[j]))
+                    return 'const int %s = %s;' % (nm, ' + '.join(terms) if terms else '0')
+                nm = self._base(('l', p.name, kinds), mk)
+                for j, (k, off) in enumerate(sh):
+                    o = self._dsum(k, off, pr) if k != 'const' else \
+                        (off if isinstance(off, int) else pr.expr(off))
+                    parts.append((o, W[j]))
+                return 'L_%s[%s + %s]' % (p.name, nm, self._off(parts, pr))
+            nd = len(sh)
+
+            def mk(nm):
+                terms = []
+                for j, (k, off) in enumerate(sh):
+                    pbk = self.pb[k] if k != 'const' else '0'
+                    terms.append('(%s)(%s - LB_%s_%d)%s' % (index_t, pbk, f.name, j,
+                                                             ' * S_%s_%d' % (f.name, j) if j < nd - 1 else ''))
+                return 'const %s* __restrict__ %s = %s + (%s);' % (
+                    c_type(f.typ), nm, _gname(f), ' + '.join(terms) if terms else '0')
+            nm = self._base(('g', f.name, kinds), mk)
+            for j, (k, off) in enumerate(sh):
+                o = self._dsum(k, off, pr) if k != 'const' else \
+                    (off if isinstance(off, int) else pr.expr(off))
+                parts.append((o, 'S_%s_%d' % (f.name, j) if j < nd - 1 else 1))
+            return '%s[%s]' % (nm, self._off(parts, pr))
 
     # a tile whose region exceeds the planned static box (possible only for
     # sizes far from the planning estimates) is skipped and reported
@@ -244,32 +351,93 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                     conds.append('%s <= %s' % (pnames[i], R(hi_id)))
         return conds
 
-    def emit_stage_value(s, target, pnames, ind, skip_box_single):
+    def emit_stage_value(out, s, target, pnames, ind, skip_box_single, item):
         """Statements assigning the value of stage s at point pnames to
-        ``target``; returns name of a bool flag set when a part applied."""
+        ``target`` and setting ``pmx_w`` when a part applied."""
+        ww = lambda t, i: out.append('  ' * i + t)
         vn = dict(zip(s.func.variables, pnames))
-        pr = Printer(vn, make_ref_cb(s), mode, lazy=True)
+        pr = Printer(vn, make_ref_cb(s, item), mode, lazy=True)
         pr._tmp = tmp_counter
         single = (len(s.parts) == 1)
         for pi, part in enumerate(s.parts):
             bc = part_cond(s, pi, part, pnames, skip_box_single and single)
-            w('if (%s) {' % (' && '.join(bc) if bc else 'true'), ind)
+            ww('if (%s) {' % (' && '.join(bc) if bc else 'true'), ind)
             inner = ind + 1
             if part.pred:
                 pro = []
                 pc = ' && '.join(pr.cond(c, pro) for c in part.pred)
                 for l in pro:
-                    w(l, inner)
-                w('if (%s) {' % pc, inner)
+                    ww(l, inner)
+                ww('if (%s) {' % pc, inner)
                 inner += 1
             pro = []
             val = pr.expr(part.expr, pro)
             for l in pro:
-                w(l, inner)
-            w('%s = %s; pmx_w = true;' % (target, val), inner)
+                ww(l, inner)
+            ww('%s = %s; pmx_w = true;' % (target, val), inner)
             if part.pred:
-                w('}', inner - 1)
-            w('}', ind)
+                ww('}', inner - 1)
+            ww('}', ind)
+
+    def emit_point(out, s, item, ind, vname, wname):
+        """One point of stage s: ``vname``/``wname`` receive value and flag."""
+        ww = lambda t, i: out.append('  ' * i + t)
+        ww('{', ind)
+        pnames = []
+        for i in range(s.nd):
+            d = item.d[i]
+            ww('const int q%d = %s + %s;' % (i, item.pb[i], d), ind + 1)
+            pnames.append('q%d' % i)
+        ww('bool pmx_w = false;', ind + 1)
+        for t in inline_closure(s):
+            tn = ['q%d' % i for i in range(t.nd)]
+            ww('%s v_%s = (%s)0;' % (c_type(t.func.typ), t.name, c_type(t.func.typ)), ind + 1)
+            ww('{', ind + 1)
+            ww('bool pmx_w = false;', ind + 2)
+            emit_stage_value(out, t, 'v_' + t.name, tn, ind + 2, False, item)
+            ww('(void)pmx_w;', ind + 2)
+            ww('}', ind + 1)
+        ww('%s v = (%s)0;' % (c_type(s.func.typ), c_type(s.func.typ)), ind + 1)
+        emit_stage_value(out, s, 'v', pnames, ind + 1, True, item)
+        ww('%s = v; %s = pmx_w;' % (vname, wname), ind + 1)
+        ww('}', ind)
+
+    def emit_store(out, s, item, ind, vname, wname):
+        ww = lambda t, i: out.append('  ' * i + t)
+        ww('if (%s) {' % wname, ind)
+        if s.needs_lds:
+            ext = plan.lds_ext(s)
+            W = [_prod(ext[j + 1:]) for j in range(s.nd)]
+
+            def mk(nm):
+                terms = ['(%s - %s) * %d' % (item.pb[j], R(s.region[j][0]), W[j]) for j in range(s.nd)]
+                return 'const int %s = %s;' % (nm, ' + '.join(terms) if terms else '0')
+            nm = item._base(('ls', s.name), mk)
+            off = item._off([(item.d[j], W[j]) for j in range(s.nd)], None)
+            ww('L_%s[%s + %s] = %s;' % (s.name, nm, off, vname), ind + 1)
+        if s.liveout:
+            f = s.func
+
+            def mkg(nm):
+                terms = ['(%s)(%s - LB_%s_%d)%s' % (index_t, item.pb[j], f.name, j,
+                                                     ' * S_%s_%d' % (f.name, j) if j < s.nd - 1 else '')
+                         for j in range(s.nd)]
+                return '%s* __restrict__ %s = %s + (%s);' % (
+                    c_type(f.typ), nm, _gname(f), ' + '.join(terms) if terms else '0')
+            nm = item._base(('gs', f.name), mkg)
+            off = item._off([(item.d[j], 'S_%s_%d' % (f.name, j) if j < s.nd - 1 else 1)
+                             for j in range(s.nd)], None)
+            st = '%s[%s] = %s;' % (nm, off, vname)
+            if s is not plan.anchor:
+                oc = []
+                for i in range(s.nd):
+                    lo, hi = s.own[i]
+                    q = '(%s + %s)' % (item.pb[i], item.d[i])
+                    oc.append('%s >= %s && %s <= %s' % (q, R(lo), q, R(hi)))
+                ww('if (%s) %s' % (' && '.join(oc), st), ind + 1)
+            else:
+                ww(st, ind + 1)
+        ww('}', ind)
 
     tmp_counter = [0]
     for ph in range(plan.nphases):
@@ -277,50 +445,92 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             if s.inline or s.phase != ph:
                 continue
             ext = s.static_ext
-            N = _prod(ext)
-            w('{ // stage %s (phase %d)' % (s.name, ph))
-            w('for (int idx = threadIdx.x; idx < %d; idx += %d) {' % (N, threads), 2)
-            rem = 'idx'
-            pnames = []
-            for i in range(s.nd):
-                st = _prod(ext[i + 1:])
-                if i < s.nd - 1:
-                    w('const int i%d = %s / %d; const int m%d = %s - i%d * %d;' % (
+            U, V = s.block
+            nd = s.nd
+            rd = nd - 2 if nd >= 2 else None
+            cd = nd - 1 if nd >= 1 else None
+            step = [1] * nd
+            cnt = list(ext)
+            if cd is not None:
+                step[cd] = V
+                cnt[cd] = -(-ext[cd] // V)
+            if rd is not None:
+                step[rd] = U
+                cnt[rd] = -(-ext[rd] // U)
+            nitems = _prod(cnt)
+            w('{ // stage %s (phase %d, %dx%d points per thread)' % (s.name, ph, U, V))
+            w('for (int it = threadIdx.x; it < %d; it += %d) {' % (nitems, threads), 2)
+            rem = 'it'
+            pb = []
+            for i in range(nd):
+                st = _prod(cnt[i + 1:])
+                if i < nd - 1:
+                    w('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
                         i, rem, st, i, rem, i, st), 3)
-                    rem = 'm%d' % i
+                    rem = 'k%d' % i
                 else:
-                    w('const int i%d = %s;' % (i, rem), 3)
-                w('const int p%d = %s + i%d;' % (i, R(s.region[i][0]), i), 3)
-                pnames.append('p%d' % i)
-            guard = ' || '.join('p%d > %s' % (i, R(s.region[i][1])) for i in range(s.nd))
+                    w('const int j%d = %s;' % (i, rem), 3)
+                w('const int b%d = %s + j%d%s;' % (i, R(s.region[i][0]), i,
+                                                    (' * %d' % step[i]) if step[i] > 1 else ''), 3)
+                pb.append('b%d' % i)
+            guard = ' || '.join('b%d > %s' % (i, R(s.region[i][1])) for i in range(nd))
             if guard:
                 w('if (%s) continue;' % guard, 3)
-            w('bool pmx_w = false;', 3)
-            for t in inline_closure(s):
-                tn = ['p%d' % i for i in range(t.nd)]
-                w('%s v_%s = (%s)0;' % (c_type(t.func.typ), t.name, c_type(t.func.typ)), 3)
-                w('{', 3)
-                w('bool pmx_w = false;', 4)
-                emit_stage_value(t, 'v_' + t.name, tn, 4, False)
-                w('(void)pmx_w;', 4)
+            T = c_type(s.func.typ)
+            fast = []
+            blocked = U * V > 1
+            pts = [(u, v) for u in range(U) for v in range(V)]
+            # fast path: the whole block lies in the region
+            fitem = None
+            body_ind = 4 if blocked else 3
+
+            def dvec(u, v):
+                d = [0] * nd
+                if rd is not None:
+                    d[rd] = u
+                if cd is not None:
+                    d[cd] = v
+                return d
+            fitem = Item(pb, dvec(0, 0))
+            for (u, v) in pts:
+                fast.append('  ' * body_ind + '%s pv_%d_%d = (%s)0; bool pw_%d_%d = false;' % (T, u, v, T, u, v))
+            for (u, v) in pts:
+                fitem.d = dvec(u, v)
+                emit_point(fast, s, fitem, body_ind, 'pv_%d_%d' % (u, v), 'pw_%d_%d' % (u, v))
+            for (u, v) in pts:
+                fitem.d = dvec(u, v)
+                emit_store(fast, s, fitem, body_ind, 'pv_%d_%d' % (u, v), 'pw_%d_%d' % (u, v))
+            slow = []
+            sitem = Item(pb, dvec('pu', 'pv') if blocked else dvec(0, 0))
+            sitem.names = fitem.names
+            sitem.decls = fitem.decls
+            if blocked:
+                slow.append('  ' * 4 + 'for (int pu = 0; pu < %d; ++pu) for (int pv = 0; pv < %d; ++pv) {' % (U, V))
+                conds = []
+                if rd is not None:
+                    conds.append('b%d + pu > %s' % (rd, R(s.region[rd][1])))
+                if cd is not None:
+                    conds.append('b%d + pv > %s' % (cd, R(s.region[cd][1])))
+                slow.append('  ' * 5 + 'if (%s) continue;' % ' || '.join(conds))
+                slow.append('  ' * 5 + '%s pv_s = (%s)0; bool pw_s = false;' % (T, T))
+                emit_point(slow, s, sitem, 5, 'pv_s', 'pw_s')
+                emit_store(slow, s, sitem, 5, 'pv_s', 'pw_s')
+                slow.append('  ' * 4 + '}')
+            for dcl in fitem.decls:
+                w(dcl, 3)
+            if blocked:
+                fc = []
+                if rd is not None:
+                    fc.append('b%d + %d <= %s' % (rd, U - 1, R(s.region[rd][1])))
+                if cd is not None:
+                    fc.append('b%d + %d <= %s' % (cd, V - 1, R(s.region[cd][1])))
+                w('if (%s) {' % ' && '.join(fc), 3)
+                L.extend(fast)
+                w('} else {', 3)
+                L.extend(slow)
                 w('}', 3)
-            w('%s v = (%s)0;' % (c_type(s.func.typ), c_type(s.func.typ)), 3)
-            emit_stage_value(s, 'v', pnames, 3, True)
-            w('if (pmx_w) {', 3)
-            if s.needs_lds:
-                w('L_%s[idx] = v;' % s.name, 4)
-            if s.liveout:
-                oc = []
-                if s is not plan.anchor:
-                    for i in range(s.nd):
-                        lo, hi = s.own[i]
-                        oc.append('p%d >= %s && p%d <= %s' % (i, R(lo), i, R(hi)))
-                st = '%s[%s] = v;' % (_gname(s.func), gindex(s.func, pnames))
-                if oc:
-                    w('if (%s) %s' % (' && '.join(oc), st), 4)
-                else:
-                    w(st, 4)
-            w('}', 3)
+            else:
+                L.extend(fast)
             w('}', 2)
             w('}')
         if ph + 1 < plan.nphases:

@@ -25,7 +25,7 @@ def gpu_available():
 
 @pytest.fixture(scope="session")
 def gpu():
-    if os.environ.get('PMX_PREBUILD'):
+    if os.environ.get('PMX_PREBUILD') or os.environ.get('PMX_EMULATE', '') not in ('', '0'):
         return True
     if not gpu_available():
         pytest.fail("GPU test selected but no HIP device is visible")

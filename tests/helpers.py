@@ -30,11 +30,22 @@ def max_rel(a, b):
     return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
 
 
+def emulating():
+    """PMX_EMULATE=1 (or =asan) runs the GPU tests on the CPU emulator of
+    the generated kernels (tests/kernel_emu.py)."""
+    import os
+    return os.environ.get('PMX_EMULATE', '') not in ('', '0')
+
+
 def gpu_build(pipe):
-    """runtime.build, or, with PMX_PREBUILD=1 (CPU-only cache warming),
-    build and skip the GPU part of the test."""
+    """runtime.build; with PMX_PREBUILD=1 (CPU-only cache warming) build
+    and skip the GPU part of the test; with PMX_EMULATE build for the CPU
+    emulator instead."""
     import os
     import pytest
+    if emulating():
+        import kernel_emu
+        return kernel_emu.build(pipe, asan=os.environ.get('PMX_EMULATE') == 'asan')
     from polymage_amd import runtime
     bp = runtime.build(pipe)
     if os.environ.get('PMX_PREBUILD'):
@@ -43,7 +54,11 @@ def gpu_build(pipe):
 
 
 def skip_if_prebuild():
+    """For tests that need a real device (not the build cache or the
+    kernel emulator)."""
     import os
     import pytest
     if os.environ.get('PMX_PREBUILD'):
         pytest.skip("prebuild only")
+    if emulating():
+        pytest.skip("needs a device, not the kernel emulator")
