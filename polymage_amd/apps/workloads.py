@@ -128,6 +128,6 @@ DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64
 
 def prebuild_default():
     from .. import runtime
-    for fn in (harris, unsharp, bilateral):
+    for fn in (harris, unsharp, bilateral, laplacian, campipe):
         w = fn(with_data=False)
         runtime.build(w.pipe)

@@ -37,10 +37,10 @@ def exhaustive(app, procs, db, runs=7):
     from . import tuner
     cands = list(tuner.default_space(app))
     t0 = time.time()
-    comp = tuner.compile_all(_workload(app), cands, procs=procs)
+    comp = tuner.compile_all(_workload(app), cands, procs=procs, verbose=True)
     t_compile = time.time() - t0
     t1 = time.time()
-    times = tuner.time_candidates(_workload(app), comp, runs=runs, db=db, app=app)
+    times = tuner.time_candidates(_workload(app), comp, runs=runs, db=db, app=app, verbose=True)
     t_time = time.time() - t1
     feats = dict((k, v[1]) for k, v in comp.items() if v[1] is not None)
     errors = dict((k, v[3]) for k, v in comp.items() if v[3])
@@ -70,8 +70,8 @@ def learned(app, procs, train_records, k=4, runs=7, db=None, seed=0):
     order, pred = costmodel.rank(model, feats)
     top = [c for c in cands if c.key in order[:k]]
     t2 = time.time()
-    comp = tuner.compile_all(_workload(app), top, procs=procs)
-    times = tuner.time_candidates(_workload(app), comp, runs=runs, db=None, app=app)
+    comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True)
+    times = tuner.time_candidates(_workload(app), comp, runs=runs, db=None, app=app, verbose=True)
     t_top = time.time() - t2
     return {'times': times, 'top': order[:k], 'pred': pred, 'losses': losses,
             'features_s': t_feat, 'train_s': t_train, 'top_s': t_top,

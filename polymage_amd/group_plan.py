@@ -36,6 +36,7 @@ from .constructs import Reference, Reduction
 from .region_ir import RegionIR, BIG
 
 LDS_LIMIT_BYTES = 160 * 1024
+CHEAP_OPS = 4          # operations of a pointwise stage recomputed per tap
 
 
 class StageInfo(object):
@@ -83,6 +84,7 @@ class GroupPlan(object):
         liveouts = pipe.liveouts(group)
         for s in self.stages:
             s.liveout = s.comp in liveouts
+        self._offset_inline()
         self._consumers()
         self._anchor()
         self._align()
@@ -107,6 +109,130 @@ class GroupPlan(object):
         for c in part.pred:
             refs += [r for r in c.collect(Reference) if r not in refs]
         return refs
+
+    def _offset_inline(self):
+        """Recompute cheap pointwise stages at every tap of their stencil
+        consumers instead of storing them.
+
+        A stage that is not a live-out, has one unconditional definition of
+        at most CHEAP_OPS operations and reads its producers only at its own
+        point (``Ixx = Ix * Ix``) is substituted into its in-group consumers
+        when one of them reads it with an offset (``Sxx`` sums Ixx over a
+        3x3 window): the consumer then reads the producers at the shifted
+        points.  The value is unchanged (the substituted expression is cast
+        to the stage's type when its C type differs), the stage disappears
+        from the group, and its producers, now read with offsets, are the
+        ones kept in LDS: Harris stores Ix and Iy instead of the three
+        products, as the reference's generated code does
+        (ipp_bench/harris/harris_function.cpp:17-21)."""
+        if self.opts.get('no_offset_inline'):
+            return
+        from .analysis import Part
+        from .expression import getType, substitute_vars, substitute_refs
+        from .expr_ast import (AbstractBinaryOpNode, AbstractUnaryOpNode, Select,
+                               InbuiltFunction, Cast)
+        changed = True
+        while changed:
+            changed = False
+            for p in self.stages:
+                if p.liveout or len(p.parts) != 1 or p.parts[0].pred:
+                    continue
+                part = p.parts[0]
+                e = part.expr
+                if e.has(Select) or e.has(InbuiltFunction):
+                    continue
+                nops = 0
+                stack = [e]
+                while stack:
+                    x = stack.pop()
+                    if isinstance(x, (AbstractBinaryOpNode, AbstractUnaryOpNode, Cast)):
+                        nops += 1
+                    if hasattr(x, 'children'):
+                        stack.extend(x.children())
+                if nops > CHEAP_OPS:
+                    continue
+                refs = e.collect(Reference)
+                if not refs or any(list(r.arguments) != list(p.func.variables) for r in refs):
+                    continue
+                # its consumers in the group; at least one reads with an offset
+                cons = []
+                offset = False
+                for c in self.stages:
+                    for cp in c.parts:
+                        for r in self._part_refs(cp):
+                            if r.objectRef is p.func:
+                                cons.append(c)
+                                if len(r.arguments) != c.nd or \
+                                        any(a is not v for a, v in zip(r.arguments, c.func.variables)):
+                                    offset = True
+                if not offset or p in cons:
+                    continue
+                if not self._taps_inside(p, part):
+                    continue
+                body = e if getType(e) is p.func.typ else Cast(p.func.typ, e)
+                for c in set(cons):
+                    new_parts = []
+                    for cp in c.parts:
+                        m = {}
+                        for r in self._part_refs(cp):
+                            if r.objectRef is p.func:
+                                m[r] = substitute_vars(body, dict(zip(p.func.variables, r.arguments)))
+                        if not m:
+                            new_parts.append(cp)
+                            continue
+                        new_parts.append(Part(cp.box, [substitute_refs(q, m) if False else q
+                                                       for q in cp.pred],
+                                              substitute_refs(cp.expr, m), cp.case_index))
+                        if any(r.objectRef is p.func for q in cp.pred for r in q.collect(Reference)):
+                            raise PlanError("offset inlining into a predicate")
+                    c.parts = new_parts
+                self.stages.remove(p)
+                self.offset_inlined = getattr(self, 'offset_inlined', []) + [p.name]
+                changed = True
+                break
+        self.by_comp = dict((s.comp, s) for s in self.stages)
+
+    def _taps_inside(self, p, part):
+        """True when every point at which an in-group consumer reads p
+        (over the consumer part's box) lies in p's Case box, for all
+        parameter values: then substituting p's expression reads nothing
+        the original program leaves undefined."""
+        from .expression import isAffine, get_affine_var_and_param_coeff, get_constant_from_expr
+        from .expr_ast import Value
+
+        def const_diff(a, b):
+            e = Value.numericToValue(a) - Value.numericToValue(b)
+            if not isAffine(e, include_div=False):
+                return None
+            if any(v != 0 for v in get_affine_var_and_param_coeff(e).values()):
+                return None
+            return get_constant_from_expr(e, affine=True)
+
+        for c in self.stages:
+            for cp in c.parts:
+                for r in self._part_refs(cp):
+                    if r.objectRef is not p.func:
+                        continue
+                    for i, arg in enumerate(r.arguments):
+                        sa = shift_access(arg, c.func.variables)
+                        if sa is None:
+                            return False
+                        plo, phi = part.box[i]
+                        plo = p.func.domain[i].lowerBound if plo is None else plo
+                        phi = p.func.domain[i].upperBound if phi is None else phi
+                        if sa[0] == 'const':
+                            lo = hi = sa[1]
+                        else:
+                            k, off = sa
+                            clo, chi = cp.box[k]
+                            clo = c.func.domain[k].lowerBound if clo is None else clo
+                            chi = c.func.domain[k].upperBound if chi is None else chi
+                            lo, hi = clo + off, chi + off
+                        d1 = const_diff(lo, plo)
+                        d2 = const_diff(phi, hi)
+                        if d1 is None or d2 is None or d1 < 0 or d2 < 0:
+                            return False
+        return True
 
     def _consumers(self):
         for s in self.stages:

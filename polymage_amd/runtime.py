@@ -86,7 +86,8 @@ def host_compile(cpp, so):
     cmd = ['g++', '-O2', '-std=c++17', '-fPIC', '-shared', '-D__HIP_PLATFORM_AMD__',
            '-I' + INCLUDE, '-I' + os.path.join(ROCM, 'include'), cpp, '-o', so,
            '-L' + HERE, '-lpmx_rt', '-L' + os.path.join(ROCM, 'lib'), '-lamdhip64',
-           '-Wl,-rpath,$ORIGIN/..', '-Wl,-rpath,$ORIGIN/../..', '-Wl,-rpath,' + os.path.join(ROCM, 'lib'),
+           '-Wl,-rpath,$ORIGIN/..', '-Wl,-rpath,$ORIGIN/../..', '-Wl,-rpath,' + HERE,
+           '-Wl,-rpath,' + os.path.join(ROCM, 'lib'),
            '-w']
     subprocess.check_call(cmd)
 

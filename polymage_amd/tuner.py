@@ -248,6 +248,10 @@ def build_candidate(workload_fn, cand, with_data=False):
                        gpu_options={'threads': cand.threads}, with_data=with_data)
 
 
+def _worker_init(cache):
+    runtime.CACHE = cache
+
+
 def _compile_one(job):
     modname, fname, kw, cand_tuple = job
     import importlib
@@ -265,9 +269,11 @@ def _compile_one(job):
         return cand.key, None, None, time.time() - t0, str(e)[:300]
 
 
-def compile_all(workload, candidates, procs=8, **kw):
+def compile_all(workload, candidates, procs=8, verbose=False, **kw):
     """Compile candidates in parallel; returns {key: (so_path, features,
-    seconds, error)}.  ``workload`` is a function of apps.workloads."""
+    seconds, error)}.  ``workload`` is a function of apps.workloads.
+    Workers are fresh interpreters ('spawn'): they never touch the GPU,
+    and the parent may already have initialised it."""
     import multiprocessing as mp
     jobs = [(workload.__module__, workload.__name__, kw,
              (c.tiles, c.group_size, c.threads)) for c in candidates]
@@ -275,11 +281,14 @@ def compile_all(workload, candidates, procs=8, **kw):
     if procs <= 1:
         res = map(_compile_one, jobs)
     else:
-        ctx = mp.get_context('fork')
-        pool = ctx.Pool(procs)
+        ctx = mp.get_context('spawn')
+        pool = ctx.Pool(procs, initializer=_worker_init, initargs=(runtime.CACHE,))
         res = pool.imap_unordered(_compile_one, jobs)
     for key, so, feats, dt, err in res:
         out[key] = (so, feats, dt, err)
+        if verbose:
+            print('  compiled %-22s %6.1fs%s' % (key, dt, ('  ERROR ' + err[:80]) if err else ''),
+                  flush=True)
     if procs > 1:
         pool.close()
         pool.join()
@@ -339,7 +348,7 @@ def time_pipeline(bp, w, runs=7, warmup=2, nsets=3):
     return ts[len(ts) // 2]
 
 
-def time_candidates(workload, compiled, runs=7, db=None, app=None, **kw):
+def time_candidates(workload, compiled, runs=7, db=None, app=None, verbose=False, **kw):
     """Time compiled candidates on the GPU; returns {key: ms}."""
     w = workload(with_data=True, **kw)
     out = {}
@@ -354,6 +363,8 @@ def time_candidates(workload, compiled, runs=7, db=None, app=None, **kw):
                 db.add({'app': app, 'key': key, 'error': str(e)[:200]})
             continue
         out[key] = ms
+        if verbose:
+            print('  timed    %-22s %8.4f ms' % (key, ms), flush=True)
         if db is not None:
             db.add({'app': app, 'key': key, 'ms': ms, 'mpix': w.mpix, 'features': feats,
                     'compile_s': dt})

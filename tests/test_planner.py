@@ -57,26 +57,47 @@ def test_exact_regions_are_tighter():
     pipe = _harris([32, 256])
     plan = plan_pipeline(pipe)[0]
     ext = dict((s.name, s.static_ext) for s in plan.stages)
-    # exact halo: Ixx/Iyy/Ixy need tile + 1 on each side, the response the tile
-    assert ext['Ixx'] == [34, 258] and ext['harris'] == [32, 256]
-    # the products read global memory only, so they are swept in 4-wide
-    # blocks and their LDS rows are padded to a multiple of 4 floats
-    assert all(s.block[1] == 4 for s in plan.stages if s.name in ('Ixx', 'Iyy', 'Ixy'))
-    assert plan.lds_bytes == 3 * 34 * 260 * 4
-    # Ix/Iy and the 3x3 sums are computed in registers
+    # the products are recomputed at each tap of the 3x3 sums, so Ix and Iy
+    # are the stored stages, on tile + 1 each side: 34 x 258, the scratch
+    # size of the reference's generated Harris at 32x256 tiles
+    # (ipp_bench/harris/harris_function.cpp:17-21, 8772 = 34 * 258 floats)
+    assert plan.offset_inlined == ['Ixx', 'Iyy', 'Ixy']
+    assert ext['Ix'] == [34, 258] and ext['Iy'] == [34, 258] and ext['harris'] == [32, 256]
+    # they read global memory only, so they are swept in 4-wide blocks and
+    # their LDS rows are padded to a multiple of 4 floats
+    assert all(s.block[1] == 4 for s in plan.stages if s.name in ('Ix', 'Iy'))
+    assert plan.lds_bytes == 2 * 34 * 260 * 4
+    # the 3x3 sums, det and trace are computed in registers
     inl = sorted(s.name for s in plan.stages if s.inline)
-    assert inl == ['Ix', 'Iy', 'Sxx', 'Sxy', 'Syy', 'det', 'trace']
+    assert inl == ['Sxx', 'Sxy', 'Syy', 'det', 'trace']
+    # the reference trapezoid computes every stage on 40 x 264 points
+    geo = reference_geometry(pipe, pipe.groups[0])
+    assert [d['scratch'] for d in geo['dims']] == [40, 264]
 
 
-def test_lds_reuse_across_phases():
-    # without inlining every stage is materialised; buffers of one storage
-    # class are reused once their last reader's phase has passed
-    pd = {}
-    out = harris_pipe(pd)
-    pipe = build(out, pd, {'R': 510, 'C': 510}, 'harris', [16, 64], group_size=11,
-                 gpu_options={'no_inline': True})
-    plan = plan_pipeline(pipe)[0]
-    offs = [s.lds_offset for s in plan.stages if s.needs_lds]
-    assert len(offs) > len(set(offs))          # some offsets shared
-    total = sum(4 * s.static_ext[0] * s.static_ext[1] for s in plan.stages if s.needs_lds)
-    assert plan.lds_bytes < total
+def test_offset_inlining_needs_taps_inside_the_case():
+    # a pointwise stage whose consumer reads it outside its Case box keeps
+    # its storage (the program reads undefined points there)
+    from polymage_amd.constructs import Variable, Parameter, Interval, Image, Function, Case, Condition
+    from polymage_amd.expr_types import Int, Float
+
+    def plan_for(lo, hi_off):
+        R = Parameter(Int, 'R')
+        x = Variable(Int, 'x')
+        img = Image(Float, 'img', [R + 2])
+        dom = Interval(Int, 0, R + 1)
+        a = Function(([x], [dom]), Float, 'a')
+        a.defn = [img(x) * 2.0]
+        b = Function(([x], [dom]), Float, 'b')
+        b.defn = [Case(Condition(x, '>=', 1) & Condition(x, '<=', R), a(x) * a(x))]
+        c = Function(([x], [dom]), Float, 'c')
+        c.defn = [Case(Condition(x, '>=', lo) & Condition(x, '<=', R - hi_off), b(x - 1) + b(x + 1))]
+        pipe = build(c, {'R': R}, {'R': 1000}, 'oi', [64], group_size=5)
+        return plan_pipeline(pipe)[0]
+
+    plan = plan_for(1, 0)                      # c reads b(0) and b(R+1)
+    assert 'b' in [s.name for s in plan.stages]
+    plan = plan_for(2, 1)                      # every tap inside b's Case
+    # b goes, then a (now read with offsets by c, unconditionally defined)
+    assert getattr(plan, 'offset_inlined', []) == ['b', 'a']
+    assert [s.name for s in plan.stages] == ['c']
