@@ -24,7 +24,7 @@ import os
 
 import numpy as np
 
-from .analysis import classify_access
+from .analysis import classify_access, eval_bexpr
 from .constructs import Reference
 from .expr_ast import (AbstractBinaryOpNode, AbstractUnaryOpNode, Select,
                        Cast, InbuiltFunction)
@@ -34,8 +34,15 @@ FEATURES = [
     'log_tile_area', 'log_total_tiles', 'overlap_ratio', 'log_lds_bytes',
     'occupancy', 'log_threads', 'ops_per_px', 'arith_intensity',
     'hbm_bytes_per_px', 'interm_bytes_per_px', 'idle_lane_frac', 'barriers',
-    'log_mpix', 'inline_frac', 'reduction_kernels',
+    'log_mpix', 'inline_frac', 'reduction_kernels', 'log_est_ns_px',
+    'waves_per_cu',
 ]
+
+# analytic time estimate: vector ALU lanes per second and HBM bytes per
+# second of one MI355X, and a per-launch overhead
+_LANE_OPS = 256 * 64 * 2.4e9
+_HBM = 8.0e12
+_LAUNCH_NS = 3000.0
 
 
 def _expr_ops(e):
@@ -66,9 +73,12 @@ def features(pipe):
     plans = plan_pipeline(pipe)
     env = pipe.param_env
     outs = pipe.outputs
+    # output pixels: the two largest extents of the first output (rows and
+    # columns, wherever the channel dimension sits)
+    exts = sorted((max(1, hi - lo + 1) for lo, hi in pipe.domain_box(outs[0])), reverse=True)
     out_px = 1
-    for lo, hi in pipe.domain_box(outs[0])[-2:]:
-        out_px *= max(1, hi - lo + 1)
+    for e in exts[:2]:
+        out_px *= e
     in_bytes = 0
     for img in pipe.inputs:
         n = 1
@@ -95,6 +105,7 @@ def features(pipe):
     barriers = 0
     inline_n = 0
     red = 0
+    waves_cu = []
     threads = int(pipe.gpu_options.get('threads', 256))
     for p in plans:
         if isinstance(p, GroupPlan):
@@ -110,23 +121,26 @@ def features(pipe):
                 tile_vol *= T
             work = 0
             for s in p.stages:
-                if s.inline:
-                    inline_n += 1
-                    continue
                 v = 1
                 for e in s.static_ext:
                     v *= e
-                work += v
-                waves = int(math.ceil(v / float(threads)))
-                idle.append(1.0 - v / float(waves * threads))
                 for part in s.parts:
-                    ops += _expr_ops(part.expr) * v
+                    ops += _expr_ops(part.expr) * v * p.total_tiles
+                if s.inline:
+                    inline_n += 1
+                    continue
+                work += v
+                U, V = getattr(s, 'block', (1, 1))
+                items = int(math.ceil(v / float(U * V)))
+                waves = int(math.ceil(items / float(threads)))
+                idle.append(1.0 - items / float(waves * threads))
             anchor_vol = 1
             for e in p.anchor.static_ext:
                 anchor_vol *= e
             overlap.append(work / float(max(1, anchor_vol)) - 1.0)
             lds = max(lds, p.lds_bytes)
             occ.append(min(8, (160 * 1024) // max(1, p.lds_bytes)) if p.lds_bytes else 8)
+            waves_cu.append(min(32, occ[-1] * threads // 64) * min(1.0, p.total_tiles / (256.0 * occ[-1])))
             barriers = max(barriers, p.nphases - 1)
             for s in p.stages:
                 if s.liveout and s.func not in outs:
@@ -138,7 +152,20 @@ def features(pipe):
             n_kernels += 1
             red += 1
             n_stages += 1
+            # the accumulated array is written once and read by the next group
+            n = 1
+            for lo, hi in pipe.domain_box(p.func):
+                n *= (hi - lo + 1)
+            interm += 2 * n * p.func.typ.nbytes()
+            # one pass over the reduction domain
+            rn = 1
+            for d in p.func.reductionDomain:
+                rn *= max(1, eval_bexpr(d.upperBound, env) - eval_bexpr(d.lowerBound, env) + 1)
+            for case in p.func.defn:
+                e = case.expression if hasattr(case, 'expression') else case
+                ops += _expr_ops(getattr(e, 'expression', e)) * rn
     ideal = in_bytes + out_bytes
+    est_ns = max(ops / _LANE_OPS, (ideal + interm) / _HBM) * 1e9 + n_kernels * _LAUNCH_NS
     f = {
         'n_kernels': n_kernels,
         'n_stages': n_stages,
@@ -151,7 +178,7 @@ def features(pipe):
         'log_lds_bytes': math.log2(max(1, lds)),
         'occupancy': float(min(occ)) if occ else 8.0,
         'log_threads': math.log2(threads),
-        'ops_per_px': ops / float(max(1, out_px)) / max(1, tot_tiles) * max(1, tot_tiles),
+        'ops_per_px': ops / float(max(1, out_px)),
         'arith_intensity': ops / float(max(1, ideal)),
         'hbm_bytes_per_px': (ideal + interm) / float(max(1, out_px)),
         'interm_bytes_per_px': interm / float(max(1, out_px)),
@@ -160,6 +187,8 @@ def features(pipe):
         'log_mpix': math.log2(max(1, out_px) / 1e6 + 1e-9),
         'inline_frac': inline_n / float(max(1, n_stages)),
         'reduction_kernels': red,
+        'log_est_ns_px': math.log(est_ns / max(1, out_px)),
+        'waves_per_cu': float(min(waves_cu)) if waves_cu else 32.0,
     }
     return f
 
