@@ -34,6 +34,10 @@ def parse():
                     choices=['unsharp', 'harris', 'bilateral', 'laplacian', 'campipe48'])
     ap.add_argument('--tiles', default=None, help='e.g. 32x256')
     ap.add_argument('--threads', type=int, default=None)
+    ap.add_argument('--gpu-opt', action='append', default=[], metavar='KEY=VALUE',
+                    help='extra code-generation option (e.g. no_stage=1, block=8,1)')
+    ap.add_argument('--schedule', default='learned', choices=['learned', 'exhaustive', 'default'],
+                    help='schedule recorded by the autotuner for this workload')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--traffic', default=os.path.join(ROOT, 'profiles', 'traffic.json'),
@@ -83,6 +87,22 @@ def cpu_baseline(w, budget_s):
             'kind': 'port',
             'sample': '%d full-size runs of the oracle C++/OpenMP port of the same pipeline '
                       '(g++ -O3 -fopenmp -march=native), min time' % runs}
+
+
+def schedule_for(args):
+    """Tiles / group size / threads for the workload: explicit flags, else
+    the cost model's pick recorded by the autotuner (polymage_amd/tuned.json,
+    written from a tuning run on this GPU), else the default tiles."""
+    from polymage_amd.apps import workloads
+    from polymage_amd import tuner
+    if args.tiles or args.threads or args.gpu_opt or args.schedule == 'default':
+        tiles = tuple(int(t) for t in args.tiles.split('x')) if args.tiles else \
+            workloads.DEFAULT_TILES[args.workload]
+        return {'tiles': tiles, 'threads': args.threads}, 'default' if not args.tiles else 'flags'
+    rec = tuner.tuned_schedule(args.workload, args.schedule)
+    if rec is None:
+        return {'tiles': workloads.DEFAULT_TILES[args.workload]}, 'default (no tuning record)'
+    return rec, '%s: %s' % (args.schedule, rec['key'])
 
 
 def batch_bench(args):
@@ -170,11 +190,15 @@ def main():
     from polymage_amd.apps import workloads
 
     gopts = {}
-    if args.threads:
-        gopts['threads'] = args.threads
-    tiles = tuple(int(t) for t in args.tiles.split('x')) if args.tiles else \
-        workloads.DEFAULT_TILES[args.workload]
-    w = getattr(workloads, args.workload)(tiles=tiles, gpu_options=gopts or None)
+    for kv in args.gpu_opt:
+        k, v = kv.split('=', 1)
+        gopts[k] = tuple(int(x) for x in v.split(',')) if ',' in v else int(v)
+    sched, sched_info = schedule_for(args)
+    tiles = sched['tiles']
+    if sched.get('threads'):
+        gopts['threads'] = sched['threads']
+    kw = {'group_size': sched['group_size']} if sched.get('group_size') else {}
+    w = getattr(workloads, args.workload)(tiles=tiles, gpu_options=gopts or None, **kw)
     bp = runtime.build(w.pipe)
 
     dev = torch.device('cuda:0')
@@ -242,7 +266,9 @@ def main():
         'dtype': 'f32',
         'data': 'synthetic (seeded smooth random field, rotated over %d HBM-resident buffer sets)' % nsets,
         'config': {'workload': '%s %s' % (w.name, 'x'.join(str(s) for s in w.out_shape)),
-                   'tiles': list(tiles), 'kernels': nk, 'schedule': 'default',
+                   'tiles': list(tiles), 'kernels': nk, 'schedule': sched_info,
+                   'gpu_options': dict((k, list(v) if isinstance(v, tuple) else v)
+                                       for k, v in gopts.items()),
                    'wall_s': round(wall, 4)},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),

@@ -127,7 +127,17 @@ DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64
 
 
 def prebuild_default():
-    from .. import runtime
-    for fn in (harris, unsharp, bilateral, laplacian, campipe):
+    """Compile (no GPU needed) the default schedules and the schedules the
+    autotuner recorded, so benchmark runs find them in the kernel cache."""
+    from .. import runtime, tuner
+    fns = {'harris': harris, 'unsharp': unsharp, 'bilateral': bilateral,
+           'laplacian': laplacian, 'campipe': campipe}
+    for name, fn in fns.items():
         w = fn(with_data=False)
         runtime.build(w.pipe)
+        for which in ('learned', 'exhaustive'):
+            rec = tuner.tuned_schedule(name, which)
+            if rec is not None:
+                w = fn(tiles=rec['tiles'], group_size=rec['group_size'],
+                       gpu_options={'threads': rec['threads']}, with_data=False)
+                runtime.build(w.pipe)

@@ -58,12 +58,37 @@ class StageInfo(object):
         self.lds_class = None
         self.needs_lds = False
 
+    def reset(self):
+        """Forget the analysis results (parts and live-out flag stay)."""
+        self.inline = False
+        self.consumers = []
+        self.align = [None] * self.nd
+        self.region = None
+        self.own = None
+        self.static_ext = None
+        self.phase = None
+        self.lds_offset = None
+        self.lds_class = None
+        self.needs_lds = False
+
     @property
     def name(self):
         return self.func.name
 
     def __repr__(self):
         return 'Stage(%s)' % self.name
+
+
+class _VirtualComp(object):
+    """Stand-in compute object of a staging copy stage (not in the
+    pipeline's graph; it exists only inside one group kernel)."""
+
+    def __init__(self, func, parts):
+        self.func = func
+        self.parts = parts
+        self.level = -1
+        self.is_reduction = False
+        self.is_const = False
 
 
 class GroupPlan(object):
@@ -79,12 +104,38 @@ class GroupPlan(object):
         comps = group.comps
         if any(c.is_reduction for c in comps) or any(c.is_const for c in comps):
             raise PlanError("not a stencil group")
+        self.staged = []
+        self._virtual = {}
+        self._build(stage=not self.opts.get('no_stage'))
+
+    def _build(self, stage):
+        pipe, group = self.pipe, self.group
+        comps = group.comps
         self.stages = [StageInfo(c, i) for i, c in enumerate(comps)]
         self.by_comp = dict((s.comp, s) for s in self.stages)
+        self._virtual = {}
+        self.staged = []
         liveouts = pipe.liveouts(group)
         for s in self.stages:
             s.liveout = s.comp in liveouts
         self._offset_inline()
+        self._analyse()
+        if stage and self._stage_inputs():
+            for s in self.stages:
+                s.reset()
+            try:
+                self._analyse()
+                self._blocks()
+                self._storage()
+                return
+            except PlanError:
+                # staging does not fit: plan without it
+                self._build(stage=False)
+                return
+        self._blocks()
+        self._storage()
+
+    def _analyse(self):
         self._consumers()
         self._anchor()
         self._align()
@@ -93,15 +144,71 @@ class GroupPlan(object):
         self._regions()
         self._sizes()
         self._phases()
-        self._blocks()
-        self._storage()
+
+    def _stage_inputs(self):
+        """Stage in LDS every out-of-group array that stages of different
+        phases read: a copy stage, computed first, holds the union of the
+        points the group reads (128-bit coalesced loads, each element read
+        from HBM once per tile) and the readers take them from LDS.
+        Unsharp's blur reads the image in the first phase and the mask
+        reads it again in the last one; without staging that second read
+        goes back to L2/HBM.  Returns True when stages were added."""
+        from .analysis import Part
+        from .constructs import Function, Interval, Variable
+        from .expr_ast import Reference as Ref
+        from .expr_types import Int
+        from .expression import substitute_refs, _subst_cond
+        phases = {}
+        for s in self.stages:
+            if s.inline:
+                continue
+            for (t, r) in self.closure_refs(s):
+                f = r.objectRef
+                if self.in_group(f) or not f.variables:
+                    continue
+                phases.setdefault(f, set()).add(s.phase)
+        cands = [f for f, ph in phases.items() if len(ph) >= 2]
+        if not cands:
+            return False
+        for g in sorted(cands, key=lambda f: f.name):
+            vs = [Variable(Int, '_st%d' % i) for i in range(len(g.variables))]
+            dom = [Interval(Int, d.lowerBound, d.upperBound) for d in g.domain]
+            vf = Function((vs, dom), g.typ, 'stg_' + g.name)
+            comp = _VirtualComp(vf, [Part([(None, None)] * len(vs), [], Ref(g, list(vs)), 0)])
+            st = StageInfo(comp, len(self.stages))
+            self._virtual[vf] = st
+            for s in self.stages:
+                new_parts = []
+                for part in s.parts:
+                    m = {}
+                    for r in self._part_refs(part):
+                        if r.objectRef is g:
+                            m[r] = Ref(vf, list(r.arguments))
+                    if not m:
+                        new_parts.append(part)
+                        continue
+                    new_parts.append(Part(part.box,
+                                          [_subst_cond(c, lambda e: substitute_refs(e, m))
+                                           for c in part.pred],
+                                          substitute_refs(part.expr, m), part.case_index))
+                s.parts = new_parts
+            self.stages.insert(0, st)
+            self.staged.append(g.name)
+        for i, s in enumerate(self.stages):
+            s.index = i
+        self.by_comp = dict((s.comp, s) for s in self.stages)
+        return True
 
     # -- structure -------------------------------------------------------------
     def in_group(self, func):
+        if func in self._virtual:
+            return True
         c = self.pipe.func_map.get(func)
         return c is not None and c in self.by_comp
 
     def stage_of(self, func):
+        if func in self._virtual:
+            return self._virtual[func]
         return self.by_comp[self.pipe.func_map[func]]
 
     def _part_refs(self, part):
@@ -494,6 +601,10 @@ class GroupPlan(object):
                 ext[s].append(int(np.max(e)) if np.size(e) else 0)
         for s in self.stages:
             s.static_ext = [max(1, e) for e in ext[s]]
+        # sampled region bounds, to recognise stages swept over equal boxes
+        self._region_samples = {}
+        for s in self.stages:
+            self._region_samples[s] = [(np.asarray(vals[l]), np.asarray(vals[h])) for (l, h) in s.region]
 
     # -- phases and storage ---------------------------------------------------------
     def _phases(self):
@@ -619,6 +730,35 @@ class GroupPlan(object):
                     if best is None or key < best[0]:
                         best = (key, (U, V))
             s.block = best[1]
+        self._sweeps()
+
+    def _sweeps(self):
+        """Stages of one phase with the same static box, block and (on every
+        sampled tile) the same region are swept by one loop: each point of
+        an item computes all of them, so loads they share (Ix and Iy read
+        the same 3x3 neighbourhood) are issued once."""
+        import numpy as np
+        self.sweeps = []
+        for ph in range(self.nphases):
+            groups = []
+            for s in self.stages:
+                if s.inline or s.phase != ph:
+                    continue
+                placed = False
+                if not self.opts.get('no_fuse') and s.nd > 0:
+                    for g in groups:
+                        t = g[0]
+                        if (t.nd, t.static_ext, t.block) != (s.nd, s.static_ext, s.block):
+                            continue
+                        if all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+                               for a, b in zip(self._region_samples[t], self._region_samples[s])):
+                            g.append(s)
+                            placed = True
+                            break
+                if not placed:
+                    groups.append([s])
+            for g in groups:
+                self.sweeps.append((ph, g))
 
     def lds_ext(self, s):
         """LDS layout extents of a stage: the static box, with the innermost

@@ -427,7 +427,12 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             nm = item._base(('gs', f.name), mkg)
             off = item._off([(item.d[j], 'S_%s_%d' % (f.name, j) if j < s.nd - 1 else 1)
                              for j in range(s.nd)], None)
-            st = '%s[%s] = %s;' % (nm, off, vname)
+            if f in final_outputs:
+                # results nobody reads back: streaming stores keep L2 for
+                # the halo re-reads of neighbouring tiles
+                st = '__builtin_nontemporal_store(%s, &%s[%s]);' % (vname, nm, off)
+            else:
+                st = '%s[%s] = %s;' % (nm, off, vname)
             if s is not plan.anchor:
                 oc = []
                 for i in range(s.nd):
@@ -440,101 +445,154 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         ww('}', ind)
 
     tmp_counter = [0]
-    for ph in range(plan.nphases):
-        for s in stages:
-            if s.inline or s.phase != ph:
-                continue
-            ext = s.static_ext
-            U, V = s.block
-            nd = s.nd
-            rd = nd - 2 if nd >= 2 else None
-            cd = nd - 1 if nd >= 1 else None
-            step = [1] * nd
-            cnt = list(ext)
-            if cd is not None:
-                step[cd] = V
-                cnt[cd] = -(-ext[cd] // V)
-            if rd is not None:
-                step[rd] = U
-                cnt[rd] = -(-ext[rd] // U)
-            nitems = _prod(cnt)
-            w('{ // stage %s (phase %d, %dx%d points per thread)' % (s.name, ph, U, V))
-            w('for (int it = threadIdx.x; it < %d; it += %d) {' % (nitems, threads), 2)
-            rem = 'it'
-            pb = []
-            for i in range(nd):
-                st = _prod(cnt[i + 1:])
-                if i < nd - 1:
-                    w('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
-                        i, rem, st, i, rem, i, st), 3)
-                    rem = 'k%d' % i
-                else:
-                    w('const int j%d = %s;' % (i, rem), 3)
-                w('const int b%d = %s + j%d%s;' % (i, R(s.region[i][0]), i,
-                                                    (' * %d' % step[i]) if step[i] > 1 else ''), 3)
-                pb.append('b%d' % i)
-            guard = ' || '.join('b%d > %s' % (i, R(s.region[i][1])) for i in range(nd))
-            if guard:
-                w('if (%s) continue;' % guard, 3)
-            T = c_type(s.func.typ)
-            fast = []
-            blocked = U * V > 1
-            pts = [(u, v) for u in range(U) for v in range(V)]
-            # fast path: the whole block lies in the region
-            fitem = None
-            body_ind = 4 if blocked else 3
+    final_outputs = set(pipe.outputs) if not plan.opts.get('temporal_stores') else set()
+    # union boxes of fused sweeps (equal on every sampled tile; the union
+    # keeps the loop correct for any tile and is covered by the guard)
+    union_lines = []
+    sweep_box = {}
+    over2 = []
+    for si, (ph, members) in enumerate(plan.sweeps):
+        s0 = members[0]
+        if len(members) == 1:
+            sweep_box[si] = [(R(lo), R(hi)) for (lo, hi) in s0.region]
+            continue
+        box = []
+        for i in range(s0.nd):
+            los = sorted(set(R(m.region[i][0]) for m in members))
+            his = sorted(set(R(m.region[i][1]) for m in members))
+            lo = los[0]
+            for x in los[1:]:
+                lo = 'pmx_imin(%s, %s)' % (lo, x)
+            hi = his[0]
+            for x in his[1:]:
+                hi = 'pmx_imax(%s, %s)' % (hi, x)
+            union_lines.append('const int sw%d_lo%d = %s; const int sw%d_hi%d = %s;' % (si, i, lo, si, i, hi))
+            box.append(('sw%d_lo%d' % (si, i), 'sw%d_hi%d' % (si, i)))
+            over2.append('(sw%d_hi%d - sw%d_lo%d + 1 > %d)' % (si, i, si, i, s0.static_ext[i]))
+        sweep_box[si] = box
+    # (emitted before the LDS views; see the guard above)
+    for l in union_lines:
+        w(l)
+    if over2:
+        w('if (%s) { if (threadIdx.x == 0) atomicOr(pmx_err, 1); return; }' % ' || '.join(over2))
 
-            def dvec(u, v):
-                d = [0] * nd
-                if rd is not None:
-                    d[rd] = u
-                if cd is not None:
-                    d[cd] = v
-                return d
-            fitem = Item(pb, dvec(0, 0))
-            for (u, v) in pts:
-                fast.append('  ' * body_ind + '%s pv_%d_%d = (%s)0; bool pw_%d_%d = false;' % (T, u, v, T, u, v))
-            for (u, v) in pts:
-                fitem.d = dvec(u, v)
-                emit_point(fast, s, fitem, body_ind, 'pv_%d_%d' % (u, v), 'pw_%d_%d' % (u, v))
-            for (u, v) in pts:
-                fitem.d = dvec(u, v)
-                emit_store(fast, s, fitem, body_ind, 'pv_%d_%d' % (u, v), 'pw_%d_%d' % (u, v))
-            slow = []
-            sitem = Item(pb, dvec('pu', 'pv') if blocked else dvec(0, 0))
-            sitem.names = fitem.names
-            sitem.decls = fitem.decls
-            if blocked:
-                slow.append('  ' * 4 + 'for (int pu = 0; pu < %d; ++pu) for (int pv = 0; pv < %d; ++pv) {' % (U, V))
-                conds = []
-                if rd is not None:
-                    conds.append('b%d + pu > %s' % (rd, R(s.region[rd][1])))
-                if cd is not None:
-                    conds.append('b%d + pv > %s' % (cd, R(s.region[cd][1])))
-                slow.append('  ' * 5 + 'if (%s) continue;' % ' || '.join(conds))
-                slow.append('  ' * 5 + '%s pv_s = (%s)0; bool pw_s = false;' % (T, T))
-                emit_point(slow, s, sitem, 5, 'pv_s', 'pw_s')
-                emit_store(slow, s, sitem, 5, 'pv_s', 'pw_s')
-                slow.append('  ' * 4 + '}')
-            for dcl in fitem.decls:
-                w(dcl, 3)
-            if blocked:
-                fc = []
-                if rd is not None:
-                    fc.append('b%d + %d <= %s' % (rd, U - 1, R(s.region[rd][1])))
-                if cd is not None:
-                    fc.append('b%d + %d <= %s' % (cd, V - 1, R(s.region[cd][1])))
-                w('if (%s) {' % ' && '.join(fc), 3)
-                L.extend(fast)
-                w('} else {', 3)
-                L.extend(slow)
-                w('}', 3)
-            else:
-                L.extend(fast)
-            w('}', 2)
-            w('}')
-        if ph + 1 < plan.nphases:
+    for si, (ph, members) in enumerate(plan.sweeps):
+        if si > 0 and plan.sweeps[si - 1][0] != ph:
             w('__syncthreads();')
+        s0 = members[0]
+        box = sweep_box[si]
+        ext = s0.static_ext
+        U, V = s0.block
+        nd = s0.nd
+        rd = nd - 2 if nd >= 2 else None
+        cd = nd - 1 if nd >= 1 else None
+        step = [1] * nd
+        cnt = list(ext)
+        if cd is not None:
+            step[cd] = V
+            cnt[cd] = -(-ext[cd] // V)
+        if rd is not None:
+            step[rd] = U
+            cnt[rd] = -(-ext[rd] // U)
+        nitems = _prod(cnt)
+        w('{ // stage %s (phase %d, %dx%d points per thread)' % (
+            ' + '.join(m.name for m in members), ph, U, V))
+        w('for (int it = threadIdx.x; it < %d; it += %d) {' % (nitems, threads), 2)
+        rem = 'it'
+        pb = []
+        for i in range(nd):
+            st = _prod(cnt[i + 1:])
+            if i < nd - 1:
+                w('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
+                    i, rem, st, i, rem, i, st), 3)
+                rem = 'k%d' % i
+            else:
+                w('const int j%d = %s;' % (i, rem), 3)
+            w('const int b%d = %s + j%d%s;' % (i, box[i][0], i,
+                                                (' * %d' % step[i]) if step[i] > 1 else ''), 3)
+            pb.append('b%d' % i)
+        guard = ' || '.join('b%d > %s' % (i, box[i][1]) for i in range(nd))
+        if guard:
+            w('if (%s) continue;' % guard, 3)
+        blocked = U * V > 1
+        pts = [(u, v) for u in range(U) for v in range(V)]
+        body_ind = 4 if blocked or len(members) > 1 else 3
+
+        def dvec(u, v):
+            d = [0] * nd
+            if rd is not None:
+                d[rd] = u
+            if cd is not None:
+                d[cd] = v
+            return d
+
+        def inside(m, lo_terms, hi_terms):
+            """C condition: coordinates lo_terms[i]..hi_terms[i] (hi None:
+            just lo_terms[i]) lie in m's region; checks the loop guard
+            already implies (b_i <= the sweep box's upper bound, which is
+            m's own) are left out."""
+            c = []
+            for i in range(nd):
+                mlo, mhi = R(m.region[i][0]), R(m.region[i][1])
+                if mlo != box[i][0]:
+                    c.append('%s >= %s' % (lo_terms[i], mlo))
+                top = hi_terms[i] if hi_terms[i] is not None else lo_terms[i]
+                if not (mhi == box[i][1] and top == 'b%d' % i):
+                    c.append('%s <= %s' % (top, mhi))
+            return c
+
+        fitem = Item(pb, dvec(0, 0))
+        fast = []
+        for mi, m in enumerate(members):
+            T = c_type(m.func.typ)
+            for (u, v) in pts:
+                fast.append('  ' * body_ind + '%s pv%d_%d_%d = (%s)0; bool pw%d_%d_%d = false;' % (
+                    T, mi, u, v, T, mi, u, v))
+        for mi, m in enumerate(members):
+            for (u, v) in pts:
+                fitem.d = dvec(u, v)
+                emit_point(fast, m, fitem, body_ind, 'pv%d_%d_%d' % (mi, u, v), 'pw%d_%d_%d' % (mi, u, v))
+        for mi, m in enumerate(members):
+            for (u, v) in pts:
+                fitem.d = dvec(u, v)
+                emit_store(fast, m, fitem, body_ind, 'pv%d_%d_%d' % (mi, u, v), 'pw%d_%d_%d' % (mi, u, v))
+        slow = []
+        sitem = Item(pb, dvec('pu', 'pv'))
+        sitem.names = fitem.names
+        sitem.decls = fitem.decls
+        need_slow = blocked or len(members) > 1
+        if need_slow:
+            slow.append('  ' * 4 + 'for (int pu = 0; pu < %d; ++pu) for (int pv = 0; pv < %d; ++pv) {' % (U, V))
+            pt = ['b%d%s' % (i, ' + pu' if i == rd else (' + pv' if i == cd else '')) for i in range(nd)]
+            for mi, m in enumerate(members):
+                conds = inside(m, pt, [None] * nd)
+                T = c_type(m.func.typ)
+                slow.append('  ' * 5 + 'if (%s) {' % (' && '.join(conds) if conds else 'true'))
+                slow.append('  ' * 6 + '%s pv_s = (%s)0; bool pw_s = false;' % (T, T))
+                emit_point(slow, m, sitem, 6, 'pv_s', 'pw_s')
+                emit_store(slow, m, sitem, 6, 'pv_s', 'pw_s')
+                slow.append('  ' * 5 + '}')
+            slow.append('  ' * 4 + '}')
+        for dcl in fitem.decls:
+            w(dcl, 3)
+        if need_slow:
+            fc = []
+            lo_t = ['b%d' % i for i in range(nd)]
+            hi_t = ['b%d + %d' % (i, (U - 1) if i == rd else ((V - 1) if i == cd else 0))
+                    if i in (rd, cd) else None for i in range(nd)]
+            for m in members:
+                for c in inside(m, lo_t, hi_t):
+                    if c not in fc:
+                        fc.append(c)
+            w('if (%s) {' % (' && '.join(fc) if fc else 'true'), 3)
+            L.extend(fast)
+            w('} else {', 3)
+            L.extend(slow)
+            w('}', 3)
+        else:
+            L.extend(fast)
+        w('}', 2)
+        w('}')
     grid_c = ' * '.join('((%s + %d) / %d)' % (_extent_c(a.func, d, base_pr), T - 1, T)
                         for (d, T) in grid) or '1'
     info = KernelInfo(kname, 'group', buffers, grid_c, threads, plan.lds_bytes, plan, reads, writes)

@@ -369,3 +369,32 @@ def time_candidates(workload, compiled, runs=7, db=None, app=None, verbose=False
             db.add({'app': app, 'key': key, 'ms': ms, 'mpix': w.mpix, 'features': feats,
                     'compile_s': dt})
     return out
+
+
+# ---------------------------------------------------------------------------
+# recorded schedules
+# ---------------------------------------------------------------------------
+
+TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuned.json')
+
+
+def parse_key(key):
+    """'T32x256_g5_w256' -> {'tiles': (32, 256), 'group_size': 5, 'threads': 256}."""
+    t, g, w = key.split('_')
+    return {'tiles': tuple(int(x) for x in t[1:].split('x')), 'group_size': int(g[1:]),
+            'threads': int(w[1:]), 'key': key}
+
+
+def tuned_schedule(app, which='learned', path=None):
+    """The schedule the autotuner recorded for ``app`` ('learned': the cost
+    model's best timed top-k candidate; 'exhaustive': the exhaustive best),
+    or None."""
+    path = path or TUNED_PATH
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(app)
+    if not rec:
+        return None
+    key = rec.get('learned_best' if which == 'learned' else 'exhaustive_best')
+    return parse_key(key) if key else None

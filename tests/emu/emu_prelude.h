@@ -18,6 +18,7 @@ extern pmx_dim3 gridDim;
 #define __shared__ static
 #define __align__(n) __attribute__((aligned(n)))
 void __syncthreads();
+#define __builtin_nontemporal_store(v, p) (*(p) = (v))
 static inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 static inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 static inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
