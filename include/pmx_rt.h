@@ -60,6 +60,12 @@ int pmx_launch(void* function, unsigned gx, unsigned gy, unsigned gz,
                unsigned bx, unsigned by, unsigned bz, unsigned shmem,
                void* stream, void** args);
 
+/* Resident workgroups of `function` per compute unit at `threads` threads
+ * and `dyn_lds` bytes of dynamic LDS (sizes persistent launches). */
+int pmx_occupancy(void* function, int threads, size_t dyn_lds, int* blocks_per_cu);
+/* Compute units of device `dev`. */
+int pmx_device_cus(int dev, int* n);
+
 /* ---- streams, events, graphs --------------------------------------------- */
 /* Streams are created at the default priority (non-blocking w.r.t. the
  * null stream); graph capture needs a stream other than the null stream. */

@@ -174,6 +174,21 @@ int pmx_launch(void* function, unsigned gx, unsigned gy, unsigned gz,
                    "hipModuleLaunchKernel");
 }
 
+int pmx_occupancy(void* function, int threads, size_t dyn_lds, int* blocks_per_cu) {
+  int n = 0;
+  int rc = hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &n, (hipFunction_t)function, threads, dyn_lds),
+                     "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+  if (rc) return rc;
+  *blocks_per_cu = n;
+  return 0;
+}
+
+int pmx_device_cus(int dev, int* n) {
+  return hip_check(hipDeviceGetAttribute(n, hipDeviceAttributeMultiprocessorCount, dev),
+                   "hipDeviceGetAttribute");
+}
+
 int pmx_stream_create(void** stream) {
   hipStream_t s;
   int rc = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");

@@ -106,7 +106,9 @@ class GroupPlan(object):
             raise PlanError("not a stencil group")
         self.staged = []
         self._virtual = {}
-        self._build(stage=not self.opts.get('no_stage'))
+        # staging trades LDS (occupancy) for fewer L2/HBM re-reads; measured
+        # slower on unsharp at 32x256, so it is an option the tuner can try
+        self._build(stage=bool(self.opts.get('stage')))
 
     def _build(self, stage):
         pipe, group = self.pipe, self.group
@@ -368,7 +370,14 @@ class GroupPlan(object):
 
     def _identity_dim(self, d):
         """True when every in-group dependence along anchor dim d is the
-        identity, so splitting it one point per tile costs no halo."""
+        identity and every stage has a dimension aligned with d, so
+        splitting it one point per tile costs no halo and no recompute (a
+        stage without such a dimension, like the demosaic stages feeding a
+        camera pipeline's colour-channel output, would be computed again by
+        every workgroup of the split)."""
+        for s in self.stages:
+            if not any(al is not None and al[0] == d for al in s.align):
+                return False
         for s in self.stages:
             for (c, part, r) in s.consumers:
                 for i, arg in enumerate(r.arguments):

@@ -428,8 +428,8 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             off = item._off([(item.d[j], 'S_%s_%d' % (f.name, j) if j < s.nd - 1 else 1)
                              for j in range(s.nd)], None)
             if f in final_outputs:
-                # results nobody reads back: streaming stores keep L2 for
-                # the halo re-reads of neighbouring tiles
+                # option: streaming stores for results nobody reads back
+                # (measured slower on unsharp, so off by default)
                 st = '__builtin_nontemporal_store(%s, &%s[%s]);' % (vname, nm, off)
             else:
                 st = '%s[%s] = %s;' % (nm, off, vname)
@@ -445,7 +445,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         ww('}', ind)
 
     tmp_counter = [0]
-    final_outputs = set(pipe.outputs) if not plan.opts.get('temporal_stores') else set()
+    final_outputs = set(pipe.outputs) if plan.opts.get('nt_stores') else set()
     # union boxes of fused sweeps (equal on every sampled tile; the union
     # keeps the loop correct for any tile and is covered by the guard)
     union_lines = []
@@ -622,11 +622,11 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines):
 
     ``<k>``       one workgroup per tile (grid = number of tiles);
     ``<k>_batch`` persistent: a fixed set of workgroups pulls (frame, tile)
-                  work items from an atomic counter until the queue over all
-                  frames is empty; frames are described by ``<k>_frame``
-                  records (buffers, first global work item, parameters).
-    Every workgroup of the persistent form leaves the loop once the counter
-    passes ``total``, so the grid always drains."""
+                  work items from 8 atomic queues (one per XCD) until all
+                  are empty; frames are described by ``<k>_frame`` records
+                  (buffers, first global work item, parameters).
+    Every workgroup of the persistent form leaves the loop once all 8
+    counters have passed their queue's end, so the grid always drains."""
     decls, names, fields = kernel_args(params, buffers, writes)
     lds = max(16, lds_bytes)
     L = ['__device__ __forceinline__ void %s_body(%s) {' % (
@@ -646,12 +646,28 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines):
              % (threads, kname, kname))
     L.append('  __shared__ __align__(16) unsigned char pmx_lds[%d];' % lds)
     L.append('  __shared__ long long pmx_work;')
+    L.append('  // 8 queues, one per XCD (workgroups are dealt round-robin to the')
+    L.append('  // XCDs): queue q holds the contiguous items [total*q/8, total*(q+1)/8),')
+    L.append('  // so neighbouring tiles share an L2; an empty home queue sends the')
+    L.append('  // workgroup on to the next ones until all 8 are empty.')
+    L.append('  const int home = (int)(blockIdx.x & 7);')
+    L.append('  int q = 0;')
     L.append('  for (;;) {')
     L.append('    __syncthreads();')
-    L.append('    if (threadIdx.x == 0) pmx_work = (long long)atomicAdd(counter, 1ULL);')
+    L.append('    if (threadIdx.x == 0) {')
+    L.append('      long long w = -1;')
+    L.append('      while (q < 8) {')
+    L.append('        const int qq = (home + q) & 7;')
+    L.append('        const long long lo = total * qq / 8, hi = total * (qq + 1) / 8;')
+    L.append('        const long long k = (long long)atomicAdd(&counter[qq], 1ULL);')
+    L.append('        if (lo + k < hi) { w = lo + k; break; }')
+    L.append('        ++q;')
+    L.append('      }')
+    L.append('      pmx_work = w;')
+    L.append('    }')
     L.append('    __syncthreads();')
     L.append('    const long long t = pmx_work;')
-    L.append('    if (t >= total) break;')
+    L.append('    if (t < 0) break;')
     L.append('    int lo = 0, hi = nframes - 1;')
     L.append('    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (F[mid].pmx_tile0 <= t) lo = mid; else hi = mid - 1; }')
     L.append('    const %s_frame& f = F[lo];' % kname)

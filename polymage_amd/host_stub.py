@@ -267,6 +267,7 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     w('  void* frames[%d] = {};' % max(1, nk))
     w('  long long total[%d] = {};' % max(1, nk))
     w('  void* counters = nullptr; void* graph = nullptr;')
+    w('  long long grid[%d] = {};' % max(1, nk))
     for p in params:
         w('  std::vector<%s> p_%s;' % (c_type(p.typ), p.name))
     for f in inputs + outputs:
@@ -332,7 +333,14 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
           % (ki, k.name))
         w('  rc = pmx_memcpy_h2d(B->frames[%d], rec_%s.data(), sizeof(%s_frame) * (size_t)n, nullptr); '
           'if (rc) { delete B; return rc; }' % (ki, k.name, k.name))
-    w('  rc = pmx_malloc(&B->counters, 8 * %d); if (rc) { delete B; return rc; }' % max(1, nk))
+    # persistent grids: as many workgroups as stay resident on the device
+    w('  { int cus = 0; rc = pmx_device_cus(B->dev, &cus); if (rc) { delete B; return rc; }')
+    for ki, k in enumerate(kernels):
+        w('    { int occ = 0; rc = pmx_occupancy(pmx_fn_batch[B->dev][%d], %d, 0, &occ); '
+          'if (rc) { delete B; return rc; } if (occ < 1) occ = 1; B->grid[%d] = (long long)occ * cus; }'
+          % (ki, k.threads, ki))
+    w('  }')
+    w('  rc = pmx_malloc(&B->counters, 64 * %d); if (rc) { delete B; return rc; }' % max(1, nk))
     w('  rc = pmx_stream_sync(nullptr); if (rc) { delete B; return rc; }')
     w('  *handle = (void*)B;')
     w('  return 0;')
@@ -340,13 +348,13 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     w('extern "C" int pipeline_%s_batch_run(void* handle, void* stream) {' % name)
     w('  pmx_batch_%s* B = (pmx_batch_%s*)handle;' % (name, name))
     w('  int rc = pmx_load(B->dev); if (rc) return rc;')
-    w('  { void* c = B->counters; int nc = %d; void* args[] = {&c, &nc};' % max(1, nk))
+    w('  { void* c = B->counters; int nc = %d; void* args[] = {&c, &nc};' % (8 * max(1, nk)))
     w('    rc = pmx_launch(pmx_zero_fn[B->dev], 1, 1, 1, 64, 1, 1, 0, stream, args); if (rc) return rc; }')
     for ki, k in enumerate(kernels):
         w('  { void* f = B->frames[%d]; int n = B->n; unsigned long long* c = (unsigned long long*)B->counters + %d; '
-          'long long t = B->total[%d];' % (ki, ki, ki))
+          'long long t = B->total[%d];' % (ki, 8 * ki, ki))
         w('    void* args[] = {&f, &n, &c, &t};')
-        w('    long long g = t < %d ? t : %d;' % (PERSISTENT_WG, PERSISTENT_WG))
+        w('    long long g = t < B->grid[%d] ? t : B->grid[%d];' % (ki, ki))
         w('    if (g > 0) { rc = pmx_launch(pmx_fn_batch[B->dev][%d], (unsigned)g, 1, 1, %d, 1, 1, 0, stream, args); if (rc) return rc; } }'
           % (ki, k.threads))
     w('  return 0;')
@@ -384,5 +392,3 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     w('  return 0;')
     w('}')
 
-
-PERSISTENT_WG = 1024

@@ -88,6 +88,8 @@ extern "C" int pmx_module_function(void*, const char* name, void** function) {
   return 0;
 }
 extern "C" int pmx_module_unload(void*) { return 0; }
+extern "C" int pmx_occupancy(void*, int, size_t, int* n) { *n = 2; return 0; }
+extern "C" int pmx_device_cus(int, int* n) { *n = 4; return 0; }
 
 struct Launch { pmx_tramp t; void** args; unsigned grid; unsigned tid; };
 static void* worker(void* p) {

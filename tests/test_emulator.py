@@ -36,13 +36,16 @@ def test_emulated_harris(R, C, tiles, block):
     assert bits_equal(g, o)
 
 
-@pytest.mark.parametrize("block", [None, (8, 1), (1, 4), (4, 4)])
-def test_emulated_unsharp_blocks(block):
+@pytest.mark.parametrize("opts", [None, {'block': (8, 1)}, {'block': (1, 4)}, {'block': (4, 4)},
+                                  {'stage': 1}, {'stage': 1, 'nt_stores': 1}, {'no_fuse': 1}])
+def test_emulated_unsharp_options(opts):
     R, C = 45, 77
     pd = {}
     out = unsharp_mask_full(pd)
-    pipe = build(out, pd, {'R': R, 'C': C}, 'unsharp', [16, 32], group_size=5,
-                 gpu_options={'block': block} if block else None)
+    pipe = build(out, pd, {'R': R, 'C': C}, 'unsharp', [16, 32], group_size=5, gpu_options=opts)
+    if opts and opts.get('stage'):
+        from polymage_amd.hip_emit import plan_pipeline
+        assert plan_pipeline(pipe)[0].staged == ['img']
     img = inputs.smooth_field(np.random.default_rng(5), R + 4, C + 4, 3)
     vals = {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0, 'img': img}
     g = np.full((3, R, C), -7.0, np.float32)
@@ -66,3 +69,29 @@ def test_emulated_kernels_under_asan():
     out = r.stdout.decode(errors='replace')
     assert r.returncode == 0, out[-3000:]
     assert '2 passed' in out, out[-3000:]
+
+
+def test_emulated_campipe_batch_queues():
+    """Frames of three sizes through the persistent per-XCD work queues
+    (emulated grid smaller than the item count, so queues are stolen
+    from) give the oracle's output for every frame."""
+    from polymage_amd.apps.campipe import camera_pipe, MATRIX_3200, MATRIX_7000
+    pd = {}
+    out = camera_pipe(pd, variant='batch')
+    g = np.random.default_rng(3)
+    sizes = [(128, 192), (160, 256), (96, 160)]
+    pipe = build(out, pd, {'R': 160, 'C': 256}, 'campipe', [16, 64], group_size=40)
+    orc = cemit.build([out], 'cam_oracle_batch')
+    recs, refs = [], []
+    for (R, C) in sizes:
+        img = g.integers(0, 1024, (R, C)).astype(np.uint16)
+        vals = {'R': R, 'C': C, 'colour_temp': 3700.0, 'gamma': 2.2, 'contrast': 50.0, 'img': img,
+                'matrix_3200': np.array(MATRIX_3200, np.float32),
+                'matrix_7000': np.array(MATRIX_7000, np.float32)}
+        o = np.zeros((3, R - 48, C - 64), np.uint8)
+        orc(dict(vals, process=o))
+        refs.append(o)
+        recs.append(dict(vals, process=np.full_like(o, 7)))
+    kernel_emu.build(pipe).run_batch(recs, 'persistent')
+    for rec, r in zip(recs, refs):
+        assert np.array_equal(rec['process'], r)
