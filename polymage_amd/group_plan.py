@@ -365,7 +365,9 @@ class GroupPlan(object):
             if k < len(self.tile_sizes) and ext > self.tile_sizes[k]:
                 self.grid_dims.append((d, int(self.tile_sizes[k])))
                 k += 1
-            elif self._identity_dim(d):
+            elif d < best.nd - 1 and self._identity_dim(d):
+                # (never the innermost dimension: splitting it would leave
+                # each workgroup a strided, uncoalesced slice of the rows)
                 self.grid_dims.append((d, 1))
 
     def _identity_dim(self, d):
@@ -698,14 +700,22 @@ class GroupPlan(object):
         forced = self.opts.get('block')
         for s in self.stages:
             s.block = (1, 1)
+            s.bdims = (s.nd - 2 if s.nd >= 2 else None, s.nd - 1 if s.nd >= 1 else None)
             if s.inline or s.nd == 0:
                 continue
             ext = s.static_ext
-            ext_c = ext[-1]
-            ext_r = ext[-2] if s.nd >= 2 else 1
+            rd, cd = s.bdims
+            if ext[-1] < 16 and s.nd >= 3:
+                # short innermost dimension (colour channels): lanes run over
+                # the two innermost dimensions, blocks along the third
+                rd = s.nd - 3
+                s.bdims = (rd, cd)
+            ext_c = ext[cd]
+            ext_r = ext[rd] if rd is not None else 1
             outer = 1
-            for e in ext[:-2]:
-                outer *= e
+            for i, e in enumerate(ext):
+                if i not in (rd, cd):
+                    outer *= e
             reads_lds = False
             rv = rh = 0
             for (t, r) in self.closure_refs(s):
@@ -715,13 +725,13 @@ class GroupPlan(object):
                     sa = shift_access(arg, t.func.variables)
                     if sa is None or sa[0] == 'const' or not isinstance(sa[1], int):
                         continue
-                    if sa[0] == s.nd - 1:
+                    if sa[0] == cd:
                         rh = max(rh, abs(sa[1]))
-                    elif s.nd >= 2 and sa[0] == s.nd - 2:
+                    elif sa[0] == rd:
                         rv = max(rv, abs(sa[1]))
             if forced is not None:
                 U, V = forced
-                U = U if s.nd >= 2 else 1
+                U = U if rd is not None else 1
                 V = V if (not reads_lds and ext_c >= 4 * V) else 1
                 s.block = (max(1, min(U, ext_r)), V)
                 continue
@@ -729,7 +739,7 @@ class GroupPlan(object):
             best = None
             for V in Vs:
                 for U in (1, 2, 4, 8):
-                    if U > 1 and (s.nd < 2 or ext_r < 2 * U) or U * V > 16:
+                    if U > 1 and (rd is None or ext_r < 2 * U) or U * V > 16:
                         continue
                     items = outer * (-(-ext_r // U)) * (-(-ext_c // V))
                     iters = -(-items // threads)
@@ -757,7 +767,7 @@ class GroupPlan(object):
                 if not self.opts.get('no_fuse') and s.nd > 0:
                     for g in groups:
                         t = g[0]
-                        if (t.nd, t.static_ext, t.block) != (s.nd, s.static_ext, s.block):
+                        if (t.nd, t.static_ext, t.block, t.bdims) != (s.nd, s.static_ext, s.block, s.bdims):
                             continue
                         if all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
                                for a, b in zip(self._region_samples[t], self._region_samples[s])):

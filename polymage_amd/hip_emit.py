@@ -476,16 +476,16 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     if over2:
         w('if (%s) { if (threadIdx.x == 0) atomicOr(pmx_err, 1); return; }' % ' || '.join(over2))
 
+    phase_code = {}     # phase -> [(comment, nitems, lines)]
     for si, (ph, members) in enumerate(plan.sweeps):
-        if si > 0 and plan.sweeps[si - 1][0] != ph:
-            w('__syncthreads();')
+        SL = []
+        w3 = lambda t, ind=3: SL.append('  ' * ind + t)
         s0 = members[0]
         box = sweep_box[si]
         ext = s0.static_ext
         U, V = s0.block
         nd = s0.nd
-        rd = nd - 2 if nd >= 2 else None
-        cd = nd - 1 if nd >= 1 else None
+        rd, cd = s0.bdims
         step = [1] * nd
         cnt = list(ext)
         if cd is not None:
@@ -495,25 +495,24 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             step[rd] = U
             cnt[rd] = -(-ext[rd] // U)
         nitems = _prod(cnt)
-        w('{ // stage %s (phase %d, %dx%d points per thread)' % (
-            ' + '.join(m.name for m in members), ph, U, V))
-        w('for (int it = threadIdx.x; it < %d; it += %d) {' % (nitems, threads), 2)
+        comment = 'stage %s (phase %d, %dx%d points per thread)' % (
+            ' + '.join(m.name for m in members), ph, U, V)
         rem = 'it'
         pb = []
         for i in range(nd):
             st = _prod(cnt[i + 1:])
             if i < nd - 1:
-                w('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
-                    i, rem, st, i, rem, i, st), 3)
+                w3('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
+                    i, rem, st, i, rem, i, st))
                 rem = 'k%d' % i
             else:
-                w('const int j%d = %s;' % (i, rem), 3)
-            w('const int b%d = %s + j%d%s;' % (i, box[i][0], i,
-                                                (' * %d' % step[i]) if step[i] > 1 else ''), 3)
+                w3('const int j%d = %s;' % (i, rem))
+            w3('const int b%d = %s + j%d%s;' % (i, box[i][0], i,
+                                                 (' * %d' % step[i]) if step[i] > 1 else ''))
             pb.append('b%d' % i)
         guard = ' || '.join('b%d > %s' % (i, box[i][1]) for i in range(nd))
         if guard:
-            w('if (%s) continue;' % guard, 3)
+            w3('if (%s) continue;' % guard)
         blocked = U * V > 1
         pts = [(u, v) for u in range(U) for v in range(V)]
         body_ind = 4 if blocked or len(members) > 1 else 3
@@ -574,7 +573,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 slow.append('  ' * 5 + '}')
             slow.append('  ' * 4 + '}')
         for dcl in fitem.decls:
-            w(dcl, 3)
+            w3(dcl)
         if need_slow:
             fc = []
             lo_t = ['b%d' % i for i in range(nd)]
@@ -584,13 +583,37 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 for c in inside(m, lo_t, hi_t):
                     if c not in fc:
                         fc.append(c)
-            w('if (%s) {' % (' && '.join(fc) if fc else 'true'), 3)
-            L.extend(fast)
-            w('} else {', 3)
-            L.extend(slow)
-            w('}', 3)
+            w3('if (%s) {' % (' && '.join(fc) if fc else 'true'))
+            SL.extend(fast)
+            w3('} else {')
+            SL.extend(slow)
+            w3('}')
         else:
-            L.extend(fast)
+            SL.extend(fast)
+        phase_code.setdefault(ph, []).append((comment, nitems, SL))
+
+    # one loop per phase over the concatenated item spaces of its sweeps, so
+    # several small sweeps share the workgroup's lanes instead of each
+    # leaving most of them idle in its last iteration
+    for ph in range(plan.nphases):
+        sws = phase_code.get(ph, [])
+        if ph > 0:
+            w('__syncthreads();')
+        if not sws:
+            continue
+        total_items = sum(n for (_, n, _) in sws)
+        w('{ // phase %d: %s' % (ph, '; '.join(c for (c, _, _) in sws)))
+        w('for (int pmx_it = threadIdx.x; pmx_it < %d; pmx_it += %d) {' % (total_items, threads), 2)
+        base = 0
+        for k, (c, n, SL) in enumerate(sws):
+            if len(sws) == 1:
+                w('{ const int it = pmx_it;', 2)
+            else:
+                w('%sif (pmx_it < %d) { const int it = pmx_it - %d;' % (
+                    '' if k == 0 else '} else ', base + n, base), 2)
+            L.extend(SL)
+            base += n
+        w('}', 2)
         w('}', 2)
         w('}')
     grid_c = ' * '.join('((%s + %d) / %d)' % (_extent_c(a.func, d, base_pr), T - 1, T)
