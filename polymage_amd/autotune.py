@@ -8,20 +8,24 @@ For every application:
    (parallel hiprtc/g++ workers, fresh kernel cache) and time each on the
    GPU; record compile and timing seconds;
 2. learned: train the MLP cost model (HIP kernels) on the timing records
-   of the *other* applications only, featurise every candidate statically
-   (planning, no compilation), compile and time only the model's top-k;
-3. report MPix/s of both winners, their ratio, and the tuning-time ratio
-   (learned / exhaustive, wall clock including compilation).
+   of the *other* applications only (training is an offline cost, reported
+   separately), featurise every candidate statically (planning, no
+   compilation), compile and time only the model's top-k;
+3. report, per application: the exhaustive winner, the learned winner, the
+   learned winner's time *in the exhaustive run* over the exhaustive best
+   (regret free of re-measurement noise) and as re-measured, and the
+   tuning-time ratio (learned online seconds / exhaustive seconds, wall
+   clock including compilation; with and without model training).
 
 Results go to ``DIR/tuning.json`` and every measurement to
-``DIR/timings.jsonl`` (the cost model's training data).
+``DIR/timings.jsonl`` (the cost model's training data).  ``--record``
+writes the learned and exhaustive schedules to ``polymage_amd/tuned.json``
+for ``bench.py`` and the kernel-cache prebuild.
 """
 
 import argparse
 import json
-import math
 import os
-import sys
 import tempfile
 import time
 
@@ -33,7 +37,7 @@ def _workload(app):
     return getattr(workloads, app)
 
 
-def exhaustive(app, procs, db, runs=7):
+def exhaustive(app, procs, db, runs=9):
     from . import tuner
     cands = list(tuner.default_space(app))
     t0 = time.time()
@@ -48,34 +52,42 @@ def exhaustive(app, procs, db, runs=7):
             'compile_s': t_compile, 'time_s': t_time, 'n': len(cands)}
 
 
-def learned(app, procs, train_records, k=4, runs=7, db=None, seed=0):
-    from . import tuner, costmodel
-    from .apps import workloads
-    t0 = time.time()
-    cands = list(tuner.default_space(app))
-    feats = {}
-    for c in cands:
-        try:
-            w = tuner.build_candidate(_workload(app), c)
-            feats[c.key] = costmodel.features(w.pipe)
-        except Exception:
-            continue
-    t_feat = time.time() - t0
+def train_model(train_records, seed=0, epochs=600):
+    from . import costmodel
     X = np.stack([costmodel.vector(r['features']) for r in train_records])
     y = np.array([costmodel.target_of(r) for r in train_records])
     model = costmodel.MLP(seed=seed)
+    t0 = time.time()
+    losses = model.fit(X, y, epochs=epochs, lr=3e-3, seed=seed)
+    return model, losses, time.time() - t0
+
+
+def learned(app, procs, model, k=4, runs=9):
+    """Online part of the learned search: features, ranking, top-k."""
+    from . import tuner, costmodel
+    t0 = time.time()
+    cands = list(tuner.default_space(app))
+    feats = tuner.features_all(_workload(app), cands, procs=procs)
+    feats = dict((key, f) for key, f in feats.items() if f is not None)
+    t_feat = time.time() - t0
     t1 = time.time()
-    losses = model.fit(X, y, epochs=600, lr=3e-3, seed=seed)
-    t_train = time.time() - t1
     order, pred = costmodel.rank(model, feats)
+    t_rank = time.time() - t1
     top = [c for c in cands if c.key in order[:k]]
     t2 = time.time()
     comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True)
     times = tuner.time_candidates(_workload(app), comp, runs=runs, db=None, app=app, verbose=True)
     t_top = time.time() - t2
-    return {'times': times, 'top': order[:k], 'pred': pred, 'losses': losses,
-            'features_s': t_feat, 'train_s': t_train, 'top_s': t_top,
-            'total_s': time.time() - t0}
+    return {'times': times, 'top': order[:k], 'pred': pred, 'features_s': t_feat,
+            'rank_s': t_rank, 'top_s': t_top, 'online_s': time.time() - t0,
+            'n_valid': len(feats)}
+
+
+def _default_key(app):
+    from .apps.workloads import DEFAULT_TILES
+    from .tuner import Candidate
+    gs = {'harris': 11, 'unsharp': 5, 'bilateral': 5, 'laplacian': 5, 'campipe': 40}[app]
+    return Candidate(DEFAULT_TILES.get(app, (32, 64)), gs, 256).key
 
 
 def main():
@@ -84,8 +96,9 @@ def main():
     ap.add_argument('--out', default='gpurun_out/tune')
     ap.add_argument('--procs', type=int, default=12)
     ap.add_argument('--k', type=int, default=4)
-    ap.add_argument('--runs', type=int, default=7)
+    ap.add_argument('--runs', type=int, default=9)
     ap.add_argument('--reuse-db', default=None, help='timings.jsonl from an earlier run')
+    ap.add_argument('--record', action='store_true', help='write polymage_amd/tuned.json')
     args = ap.parse_args()
     from .tuner import TimingDB
     os.makedirs(args.out, exist_ok=True)
@@ -100,52 +113,80 @@ def main():
         print('exhaustive', app, flush=True)
         ex[app] = exhaustive(app, args.procs, db, runs=args.runs)
         best = min(ex[app]['times'].items(), key=lambda kv: kv[1])
-        print('  %d candidates, best %s %.4f ms, compile %.1fs, timing %.1fs' % (
-            ex[app]['n'], best[0], best[1], ex[app]['compile_s'], ex[app]['time_s']), flush=True)
+        print('  %d candidates (%d timed), best %s %.4f ms, compile %.1fs, timing %.1fs' % (
+            ex[app]['n'], len(ex[app]['times']), best[0], best[1], ex[app]['compile_s'],
+            ex[app]['time_s']), flush=True)
     records = [r for r in db.load() if 'ms' in r]
     if args.reuse_db:
-        records += [json.loads(l) for l in open(args.reuse_db) if l.strip()]
+        with open(args.reuse_db) as f:
+            records += [json.loads(l) for l in f if l.strip()]
     report = {}
     for app in apps:
         train = [r for r in records if r['app'] != app and r.get('features')]
         print('learned', app, 'trained on', len(train), 'records', flush=True)
+        model, losses, t_train = train_model(train)
         # fresh cache again so the learned path pays its own compiles
         runtime.CACHE = tempfile.mkdtemp(prefix='pmx_kc_')
-        le = learned(app, args.procs, train, k=args.k, runs=args.runs)
+        le = learned(app, args.procs, model, k=args.k, runs=args.runs)
         w = _workload(app)(with_data=False)
-        ex_best_key, ex_best = min(ex[app]['times'].items(), key=lambda kv: kv[1])
+        ex_times = ex[app]['times']
+        ex_best_key, ex_best = min(ex_times.items(), key=lambda kv: kv[1])
         le_best_key, le_best = min(le['times'].items(), key=lambda kv: kv[1])
+        # the model's pick judged on the exhaustive run's own measurements
+        in_ex = [ex_times[kk] for kk in le['top'] if kk in ex_times]
+        le_same = min(in_ex) if in_ex else None
         ex_total = ex[app]['compile_s'] + ex[app]['time_s']
+        rank_order = sorted(ex_times, key=ex_times.get)
         rep = {
-            'candidates': ex[app]['n'],
+            'candidates': ex[app]['n'], 'timed': len(ex_times),
             'exhaustive_best': ex_best_key, 'exhaustive_ms': ex_best,
             'exhaustive_mpix_s': w.mpix / (ex_best / 1e3),
             'learned_best': le_best_key, 'learned_ms': le_best,
             'learned_mpix_s': w.mpix / (le_best / 1e3),
             'learned_over_exhaustive': ex_best / le_best,
-            'exhaustive_tuning_s': ex_total, 'learned_tuning_s': le['total_s'],
-            'tuning_time_ratio': le['total_s'] / ex_total,
+            'learned_over_exhaustive_same_run': (ex_best / le_same) if le_same else None,
             'learned_top': le['top'],
-            'rank_of_exhaustive_best_in_model': (le['top'].index(ex_best_key)
-                                                 if ex_best_key in le['top'] else None),
-            'losses': le['losses'],
-            'default_ms': ex[app]['times'].get(_default_key(app)),
-            'all_times_ms': ex[app]['times'],
+            'learned_top_ranks_in_exhaustive': [rank_order.index(kk) if kk in rank_order else None
+                                                for kk in le['top']],
+            'exhaustive_tuning_s': ex_total,
+            'learned_online_s': le['online_s'], 'learned_train_s': t_train,
+            'tuning_time_ratio': le['online_s'] / ex_total,
+            'tuning_time_ratio_with_training': (le['online_s'] + t_train) / ex_total,
+            'learned_detail': {'features_s': le['features_s'], 'rank_s': le['rank_s'],
+                               'top_s': le['top_s'], 'n_valid': le['n_valid']},
+            'losses': losses,
+            'default_ms': ex_times.get(_default_key(app)),
+            'all_times_ms': ex_times,
             'errors': ex[app]['errors'],
         }
         report[app] = rep
-        print('  learned %s %.4f ms vs exhaustive %.4f ms -> %.3f; tuning %.1fs vs %.1fs (%.3f)' % (
-            le_best_key, le_best, ex_best, rep['learned_over_exhaustive'], le['total_s'],
-            ex_total, rep['tuning_time_ratio']), flush=True)
+        print('  learned %s %.4f ms vs exhaustive %.4f ms -> %.3f (same run %s); '
+              'tuning %.1fs (+%.1fs training) vs %.1fs (%.3f)' % (
+                  le_best_key, le_best, ex_best, rep['learned_over_exhaustive'],
+                  '%.3f' % rep['learned_over_exhaustive_same_run']
+                  if rep['learned_over_exhaustive_same_run'] else '-',
+                  le['online_s'], t_train, ex_total, rep['tuning_time_ratio']), flush=True)
         with open(os.path.join(args.out, 'tuning.json'), 'w') as f:
             json.dump(report, f, indent=1)
+    if args.record:
+        record(report)
 
 
-def _default_key(app):
-    from .apps.workloads import DEFAULT_TILES
-    from .tuner import Candidate
-    gs = {'harris': 11, 'unsharp': 5, 'bilateral': 5, 'laplacian': 5, 'campipe': 5}[app]
-    return Candidate(DEFAULT_TILES.get(app, (32, 64)), gs, 256).key
+def record(report, path=None):
+    """Keep the chosen schedules (and the headline ratios) in tuned.json."""
+    from .tuner import TUNED_PATH
+    path = path or TUNED_PATH
+    keep = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            keep = json.load(f)
+    for app, r in report.items():
+        keep[app] = dict((k, r[k]) for k in (
+            'exhaustive_best', 'exhaustive_ms', 'learned_best', 'learned_ms',
+            'learned_over_exhaustive', 'learned_over_exhaustive_same_run',
+            'tuning_time_ratio', 'tuning_time_ratio_with_training', 'candidates', 'timed'))
+    with open(path, 'w') as f:
+        json.dump(keep, f, indent=1, sort_keys=True)
 
 
 if __name__ == '__main__':

@@ -229,7 +229,7 @@ class GroupPlan(object):
         when one of them reads it with an offset (``Sxx`` sums Ixx over a
         3x3 window): the consumer then reads the producers at the shifted
         points.  The value is unchanged (the substituted expression is cast
-        to the stage's type when its C type differs), the stage disappears
+        to the stage's type), the stage disappears
         from the group, and its producers, now read with offsets, are the
         ones kept in LDS: Harris stores Ix and Iy instead of the three
         products, as the reference's generated code does
@@ -237,7 +237,7 @@ class GroupPlan(object):
         if self.opts.get('no_offset_inline'):
             return
         from .analysis import Part
-        from .expression import getType, substitute_vars, substitute_refs
+        from .expression import substitute_vars, substitute_refs
         from .expr_ast import (AbstractBinaryOpNode, AbstractUnaryOpNode, Select,
                                InbuiltFunction, Cast)
         changed = True
@@ -278,7 +278,9 @@ class GroupPlan(object):
                     continue
                 if not self._taps_inside(p, part):
                     continue
-                body = e if getType(e) is p.func.typ else Cast(p.func.typ, e)
+                # always cast: literals print as C doubles, so the C type of
+                # the expression can be wider than the DSL type says
+                body = Cast(p.func.typ, e)
                 for c in set(cons):
                     new_parts = []
                     for cp in c.parts:

@@ -197,6 +197,8 @@ class Pipeline(object):
         self.bounds_check()
         self.inline_pass()
         self._build_comps()
+        if not (self._gpu_options or {}).get('no_gather_inline'):
+            self.gather_inline_pass()
         self._groups = [Group([c]) for c in self._comps]
         for g in self._groups:
             g.comps[0].group = g
@@ -391,6 +393,97 @@ class Pipeline(object):
                         if lo_i < pdom[i][0] or hi_i > pdom[i][1]:
                             raise TypeError("Reference out of domain",
                                             f.name, str(r), (lo_i, hi_i), pdom[i])
+
+    # -- gather inlining ---------------------------------------------------------
+    GATHER_OPS = 48
+    GATHER_REFS = 64
+
+    def gather_inline_pass(self):
+        """Substitute a function into the consumers that read it at a
+        data-dependent index.
+
+        The local Laplacian blends, per pixel, two of K intensity levels of
+        each Laplacian pyramid level (``lPyramid(k, x, y)`` with ``k`` computed
+        from the pixel's gray value).  Materialising a level for all K values
+        of k writes and reads K full planes, of which each pixel uses two.
+        A function that is not a live-out, has a single unconditional
+        definition of at most GATHER_OPS operations and is read by some
+        consumer with a data-dependent ('*') argument is substituted into
+        those references (cast to its own type, so values are unchanged);
+        its other consumers keep reading
+        it.  Repeats until nothing changes, so chains (the upsampled level
+        read by the blend, the coarser level read by the upsampling)
+        collapse into reads of the coarser stored level at the two needed k."""
+        from .expr_ast import (AbstractBinaryOpNode, AbstractUnaryOpNode, Select,
+                               InbuiltFunction, Cast)
+        for _ in range(64):
+            done = True
+            for comp in list(self._comps):
+                P = comp.func
+                if P in self._outputs or comp.is_reduction or comp.is_const:
+                    continue
+                body = P.defn
+                if len(body) != 1 or isinstance(body[0], Case):
+                    continue
+                e = body[0]
+                nops = 0
+                stack = [e]
+                while stack:
+                    x = stack.pop()
+                    if isinstance(x, (AbstractBinaryOpNode, AbstractUnaryOpNode, Cast, Select,
+                                      InbuiltFunction)):
+                        nops += 1
+                    if hasattr(x, 'children'):
+                        stack.extend(x.children())
+                if nops > self.GATHER_OPS:
+                    continue
+                prefs = len(e.collect(Reference))
+                # only functions that read their producers at their own
+                # resolution or coarser (pointwise, shifted, upsampled, table
+                # look-ups): recomputing a downsampling stage per use would
+                # redo the finer level's work for every read
+                fine = False
+                for r in e.collect(Reference):
+                    for a in r.arguments:
+                        acc = classify_access(a, P.variables)
+                        if acc.kind == 'var' and abs(acc.slope) > 1:
+                            fine = True
+                if fine:
+                    continue
+                # always cast: literals print as C doubles, so the C type of
+                # the expression can be wider than the DSL type says
+                sub = Cast(P.typ, e)
+                hit = False
+                for ch in comp.children:
+                    cf = ch.func
+                    if ch.is_reduction:
+                        continue
+                    cond_refs = set()
+                    for case in cf.defn:
+                        if isinstance(case, Case):
+                            cond_refs |= set(id(r) for r in case.condition.collect(Reference))
+                    m = {}
+                    for r in cf.getObjects(Reference):
+                        if r.objectRef is not P or id(r) in cond_refs:
+                            continue
+                        if any(classify_access(a, cf.variables).kind == '*' for a in r.arguments):
+                            m[r] = substitute_vars(sub, dict(zip(P.variables, r.arguments)))
+                    # bound the growth: each substituted reference brings
+                    # P's references along (a pyramid would otherwise
+                    # unfold level by level into its finest level)
+                    total = sum(len(case.collect(Reference)) for case in cf.defn)
+                    if m and total + len(m) * (prefs - 1) > self.GATHER_REFS:
+                        m = {}
+                    if m:
+                        cf.replace_refs(m)
+                        hit = True
+                if hit:
+                    self.gather_inlined = getattr(self, 'gather_inlined', []) + [P.name]
+                    self._build_comps()
+                    done = False
+                    break
+            if done:
+                return
 
     # -- inline pass (inline.py:29-199) ----------------------------------------
     def inline_pass(self):

@@ -245,6 +245,7 @@ class ReductionPlan(object):
         f = self.func
         nd = len(self.fvars)
         owned = {}
+        self.pre_size = {}
         ok = True
         for j, rv in enumerate(self.rvars):
             dim = None
@@ -269,6 +270,9 @@ class ReductionPlan(object):
                 ok = False
                 break
             owned[rv] = dim[0]
+            # cells own k consecutive values of rv when the access is rv/k + c
+            if acc.slope.numerator == 1 and acc.slope.denominator > 1:
+                self.pre_size[rv] = int(acc.slope.denominator)
         env = self.pipe.param_env
         self.free_dims = [d for d in range(nd) if d not in owned.values()]
         nacc = 1
@@ -283,6 +287,13 @@ class ReductionPlan(object):
     def describe(self):
         return 'group %d: reduction %s (%s, %d accumulators per cell)' % (
             self.gindex, self.func.name, 'owner cells' if self.owner else 'serial', self.nacc)
+
+
+def _prod_int(xs):
+    n = 1
+    for x in xs:
+        n *= x
+    return n
 
 
 def _reads_of_reduction(plan):
@@ -436,11 +447,6 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
             w('{ int a = %s - 1, b = %s;' % (rlo, rhi))
             w('  while (a < b) { const int m = a + (b - a + 1) / 2; if ((%s) <= c%d) a = m; else b = m - 1; }' % (fx, d))
             w('  hi%d = a; }' % j)
-        ind = 1
-        for j in range(len(plan.rvars)):
-            w('for (int r%d = lo%d; r%d <= hi%d; ++r%d) {' % (j, j, j, j, j), ind)
-            ind += 1
-
         def acc_slot_owner():
             nm = dict((d, 'q%d' % d) for d in free)
             return 'acc[%s]' % acc_index(nm)
@@ -448,10 +454,34 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
 
         def owned_ok():
             return ' && '.join('q%d == c%d' % (d, d) for d in od)
+
+        ks = [plan.pre_size.get(rv) for rv in plan.rvars]
+        fast = all(ks) and _prod_int(ks) <= 256
+        if fast:
+            # interior cells own a full k0 x k1 x ... pre-image: constant trip
+            # counts, fully unrolled, so all its loads issue up front (the
+            # order of the accumulations, hence every sum, is unchanged)
+            w('if (%s) {' % ' && '.join('hi%d - lo%d + 1 == %d' % (j, j, k) for j, k in enumerate(ks)))
+            ind = 2
+            for j, k in enumerate(ks):
+                w('#pragma unroll', ind)
+                w('for (int i%d = 0; i%d < %d; ++i%d) { const int r%d = lo%d + i%d;' % (j, j, k, j, j, j, j), ind)
+                ind += 1
+            red_body(ind, lambda: '')
+            for j in reversed(range(len(plan.rvars))):
+                ind -= 1
+                w('}', ind)
+            w('} else {')
+        ind = 2 if fast else 1
+        for j in range(len(plan.rvars)):
+            w('for (int r%d = lo%d; r%d <= hi%d; ++r%d) {' % (j, j, j, j, j), ind)
+            ind += 1
         red_body(ind, owned_ok)
         for j in reversed(range(len(plan.rvars))):
             ind -= 1
             w('}', ind)
+        if fast:
+            w('}')
         # write out the cell's slice of the grid
         w('for (int a = 0; a < %d; ++a) {' % plan.nacc)
         rem = 'a'

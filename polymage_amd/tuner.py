@@ -223,22 +223,23 @@ class Space(object):
         return len(self.candidates)
 
 
+POW2_TILES = [(a, b) for a in (8, 16, 32, 64, 128, 256, 512) for b in (8, 16, 32, 64, 128, 256, 512)]
+
+
 def default_space(app):
-    tiles2 = [(8, 64), (8, 128), (8, 256), (16, 64), (16, 128), (16, 256), (32, 64),
-              (32, 128), (32, 256), (64, 64), (64, 128), (4, 256)]
+    """Tile sizes {8..512}^2 (powers of two, as the benchmark configuration
+    states) times the application's group sizes times workgroup sizes.
+    Schedules that do not fit (LDS) fail at planning and are skipped."""
     if app == 'unsharp':
-        return Space(tiles2, [1, 5], (128, 256, 512))
+        return Space(POW2_TILES, [1, 5], (128, 256, 512))
     if app == 'harris':
-        return Space(tiles2, [1, 5, 11], (256,))
+        return Space(POW2_TILES, [1, 5, 11], (256, 512))
     if app == 'bilateral':
-        return Space([(16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (16, 64), (64, 128),
-                      (8, 64)], [1, 5], (128, 256))
+        return Space(POW2_TILES, [1, 5], (128, 256))
     if app == 'laplacian':
-        return Space([(16, 32), (16, 64), (32, 32), (32, 64), (8, 64), (32, 128), (16, 128)],
-                     [1, 3, 5, 8], (256,))
+        return Space(POW2_TILES, [1, 3, 5, 8], (256,))
     if app == 'campipe':
-        return Space([(8, 64), (16, 64), (16, 128), (32, 64), (8, 128), (32, 128), (16, 256)],
-                     [3, 5, 12, 40], (256,))
+        return Space(POW2_TILES, [3, 5, 12, 40], (256,))
     raise KeyError(app)
 
 
@@ -250,6 +251,41 @@ def build_candidate(workload_fn, cand, with_data=False):
 
 def _worker_init(cache):
     runtime.CACHE = cache
+
+
+def _features_one(job):
+    modname, fname, kw, cand_tuple = job
+    import importlib
+    mod = importlib.import_module(modname)
+    fn = getattr(mod, fname)
+    cand = Candidate(*cand_tuple)
+    try:
+        w = build_candidate(lambda **k: fn(**dict(kw, **k)), cand)
+        from .costmodel import features
+        return cand.key, features(w.pipe), None
+    except Exception as e:
+        return cand.key, None, str(e)[:300]
+
+
+def features_all(workload, candidates, procs=8, **kw):
+    """Static features of every candidate (planning only, no compilation),
+    in parallel worker processes; {key: features or None}."""
+    import multiprocessing as mp
+    jobs = [(workload.__module__, workload.__name__, kw,
+             (c.tiles, c.group_size, c.threads)) for c in candidates]
+    out = {}
+    if procs <= 1:
+        res = map(_features_one, jobs)
+    else:
+        ctx = mp.get_context('spawn')
+        pool = ctx.Pool(procs, initializer=_worker_init, initargs=(runtime.CACHE,))
+        res = pool.imap_unordered(_features_one, jobs, chunksize=4)
+    for key, feats, err in res:
+        out[key] = feats
+    if procs > 1:
+        pool.close()
+        pool.join()
+    return out
 
 
 def _compile_one(job):
@@ -315,7 +351,7 @@ class TimingDB(object):
             return [json.loads(l) for l in f if l.strip()]
 
 
-def time_pipeline(bp, w, runs=7, warmup=2, nsets=3):
+def time_pipeline(bp, w, runs=9, warmup=2, nsets=4):
     """Median milliseconds per pipeline call with device-resident data."""
     import numpy as np
     import torch
@@ -348,7 +384,7 @@ def time_pipeline(bp, w, runs=7, warmup=2, nsets=3):
     return ts[len(ts) // 2]
 
 
-def time_candidates(workload, compiled, runs=7, db=None, app=None, verbose=False, **kw):
+def time_candidates(workload, compiled, runs=9, db=None, app=None, verbose=False, **kw):
     """Time compiled candidates on the GPU; returns {key: ms}."""
     w = workload(with_data=True, **kw)
     out = {}

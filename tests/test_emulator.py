@@ -95,3 +95,30 @@ def test_emulated_campipe_batch_queues():
     kernel_emu.build(pipe).run_batch(recs, 'persistent')
     for rec, r in zip(recs, refs):
         assert np.array_equal(rec['process'], r)
+
+
+def test_recomputed_stage_keeps_its_float_rounding():
+    """A pointwise stage recomputed at its consumer's taps (offset
+    inlining) computes in double (C literals) and is stored as float; the
+    substituted expression must be rounded to float at each tap exactly as
+    the stored value is, or sums differ in the last bit."""
+    from polymage_amd.constructs import Variable, Parameter, Interval, Image, Function, Case, Condition
+    from polymage_amd.expr_types import Int, Float
+    from polymage_amd.hip_emit import plan_pipeline
+    R = Parameter(Int, 'R')
+    x, y = Variable(Int, 'x'), Variable(Int, 'y')
+    img = Image(Float, 'img', [R + 2, R + 2])
+    dom = Interval(Int, 0, R + 1)
+    a = Function(([x, y], [dom, dom]), Float, 'a')
+    a.defn = [img(x, y) * 0.1]
+    cond = Condition(x, '>=', 1) & Condition(x, '<=', R) & Condition(y, '>=', 1) & Condition(y, '<=', R)
+    b = Function(([x, y], [dom, dom]), Float, 'b')
+    b.defn = [Case(cond, a(x - 1, y) + a(x + 1, y) + a(x, y - 1) + a(x, y + 1))]
+    pipe = build(b, {'R': R}, {'R': 61}, 'ri', [16, 32], group_size=5)
+    assert plan_pipeline(pipe)[0].offset_inlined == ['a']
+    v = np.random.default_rng(9).random((63, 63)).astype(np.float32)
+    g = np.zeros((63, 63), np.float32)
+    o = g.copy()
+    kernel_emu.build(pipe).run_host({'R': 61, 'img': v, 'b': g})
+    cemit.build([b], 'ri_oracle')({'R': 61, 'img': v, 'b': o})
+    assert bits_equal(g, o)
