@@ -7,15 +7,17 @@ For every application:
 1. exhaustive: compile every candidate of ``tuner.default_space(app)``
    (parallel hiprtc/g++ workers, fresh kernel cache) and time each on the
    GPU; record compile and timing seconds;
-2. learned: train the MLP cost model (HIP kernels) on the timing records
-   of the *other* applications only (training is an offline cost, reported
-   separately), featurise every candidate statically (planning, no
-   compilation), compile and time only the model's top-k;
+2. learned: featurise every candidate statically (planning, no
+   compilation), rank them with an ensemble of MLP cost models (HIP
+   kernels) trained on the timing records of the *other* applications,
+   compile and time the top k, refit with those k measurements, time the
+   next k (two rounds by default), keep the fastest measured;
 3. report, per application: the exhaustive winner, the learned winner, the
    learned winner's time *in the exhaustive run* over the exhaustive best
    (regret free of re-measurement noise) and as re-measured, and the
-   tuning-time ratio (learned online seconds / exhaustive seconds, wall
-   clock including compilation; with and without model training).
+   tuning-time ratio (learned seconds / exhaustive seconds, wall clock
+   including compilation and model training) and the fraction of
+   candidates timed.
 
 Results go to ``DIR/tuning.json`` and every measurement to
 ``DIR/timings.jsonl`` (the cost model's training data).  ``--record``
@@ -52,35 +54,51 @@ def exhaustive(app, procs, db, runs=9):
             'compile_s': t_compile, 'time_s': t_time, 'n': len(cands)}
 
 
-def train_model(train_records, seed=0, epochs=600):
-    from . import costmodel
-    X = np.stack([costmodel.vector(r['features']) for r in train_records])
-    y = np.array([costmodel.target_of(r) for r in train_records])
-    model = costmodel.MLP(seed=seed)
-    t0 = time.time()
-    losses = model.fit(X, y, epochs=epochs, lr=3e-3, seed=seed)
-    return model, losses, time.time() - t0
-
-
-def learned(app, procs, model, k=4, runs=9):
-    """Online part of the learned search: features, ranking, top-k."""
+def learned(app, procs, train_records, k=6, rounds=2, runs=9, dup=20):
+    """The learned search: rank every candidate by the cost model trained on
+    the other applications' timings, compile and time the top k, refit with
+    those k measurements of this application (weighted by repetition), time
+    the next k, and keep the fastest measured.  Returns timings and the
+    split of its wall-clock time."""
     from . import tuner, costmodel
     t0 = time.time()
     cands = list(tuner.default_space(app))
     feats = tuner.features_all(_workload(app), cands, procs=procs)
     feats = dict((key, f) for key, f in feats.items() if f is not None)
     t_feat = time.time() - t0
-    t1 = time.time()
-    order, pred = costmodel.rank(model, feats)
-    t_rank = time.time() - t1
-    top = [c for c in cands if c.key in order[:k]]
-    t2 = time.time()
-    comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True)
-    times = tuner.time_candidates(_workload(app), comp, runs=runs, db=None, app=app, verbose=True)
-    t_top = time.time() - t2
-    return {'times': times, 'top': order[:k], 'pred': pred, 'features_s': t_feat,
-            'rank_s': t_rank, 'top_s': t_top, 'online_s': time.time() - t0,
-            'n_valid': len(feats)}
+    keys = list(feats)
+    Xc = np.stack([costmodel.vector(feats[kk]) for kk in keys])
+    X0 = np.stack([costmodel.vector(r['features']) for r in train_records])
+    y0 = costmodel.relative_targets(train_records)
+    times = {}
+    t_train = 0.0
+    t_top = 0.0
+    tops = []
+    for rnd in range(rounds):
+        X, y = X0, y0
+        if times:
+            mk = [kk for kk in keys if kk in times]
+            ym = np.log(np.array([times[kk] for kk in mk]))
+            ym = np.minimum(ym - np.median(ym), 1.0)
+            X = np.vstack([X0] + [np.stack([costmodel.vector(feats[kk]) for kk in mk])] * dup)
+            y = np.concatenate([y0] + [ym] * dup)
+        t1 = time.time()
+        model = costmodel.Ensemble()
+        model.fit(X, y)
+        pred = model.predict(Xc)
+        t_train += time.time() - t1
+        order = [keys[i] for i in np.argsort(pred) if keys[i] not in times]
+        top_keys = order[:k]
+        tops.append(top_keys)
+        top = [c for c in cands if c.key in top_keys]
+        t2 = time.time()
+        comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True)
+        times.update(tuner.time_candidates(_workload(app), comp, runs=runs, db=None, app=app,
+                                           verbose=True))
+        t_top += time.time() - t2
+    return {'times': times, 'top': [kk for t in tops for kk in t], 'rounds': tops,
+            'features_s': t_feat, 'train_s': t_train, 'top_s': t_top,
+            'online_s': time.time() - t0, 'n_valid': len(feats)}
 
 
 def _default_key(app):
@@ -95,7 +113,8 @@ def main():
     ap.add_argument('--apps', default='unsharp,harris,bilateral,laplacian,campipe')
     ap.add_argument('--out', default='gpurun_out/tune')
     ap.add_argument('--procs', type=int, default=12)
-    ap.add_argument('--k', type=int, default=4)
+    ap.add_argument('--k', type=int, default=6)
+    ap.add_argument('--rounds', type=int, default=2)
     ap.add_argument('--runs', type=int, default=9)
     ap.add_argument('--reuse-db', default=None, help='timings.jsonl from an earlier run')
     ap.add_argument('--record', action='store_true', help='write polymage_amd/tuned.json')
@@ -124,10 +143,10 @@ def main():
     for app in apps:
         train = [r for r in records if r['app'] != app and r.get('features')]
         print('learned', app, 'trained on', len(train), 'records', flush=True)
-        model, losses, t_train = train_model(train)
         # fresh cache again so the learned path pays its own compiles
         runtime.CACHE = tempfile.mkdtemp(prefix='pmx_kc_')
-        le = learned(app, args.procs, model, k=args.k, runs=args.runs)
+        le = learned(app, args.procs, train, k=args.k, rounds=args.rounds, runs=args.runs)
+        t_train = le['train_s']
         w = _workload(app)(with_data=False)
         ex_times = ex[app]['times']
         ex_best_key, ex_best = min(ex_times.items(), key=lambda kv: kv[1])
@@ -149,23 +168,24 @@ def main():
             'learned_top_ranks_in_exhaustive': [rank_order.index(kk) if kk in rank_order else None
                                                 for kk in le['top']],
             'exhaustive_tuning_s': ex_total,
-            'learned_online_s': le['online_s'], 'learned_train_s': t_train,
+            'learned_s': le['online_s'], 'learned_train_s': t_train,
             'tuning_time_ratio': le['online_s'] / ex_total,
-            'tuning_time_ratio_with_training': (le['online_s'] + t_train) / ex_total,
-            'learned_detail': {'features_s': le['features_s'], 'rank_s': le['rank_s'],
-                               'top_s': le['top_s'], 'n_valid': le['n_valid']},
-            'losses': losses,
+            'timed_ratio': len(le['times']) / float(max(1, len(ex_times))),
+            'learned_detail': {'features_s': le['features_s'], 'train_s': le['train_s'],
+                               'top_s': le['top_s'], 'n_valid': le['n_valid'],
+                               'rounds': le['rounds']},
             'default_ms': ex_times.get(_default_key(app)),
             'all_times_ms': ex_times,
             'errors': ex[app]['errors'],
         }
         report[app] = rep
         print('  learned %s %.4f ms vs exhaustive %.4f ms -> %.3f (same run %s); '
-              'tuning %.1fs (+%.1fs training) vs %.1fs (%.3f)' % (
+              'tuning %.1fs (of which training %.1fs) vs %.1fs (%.3f); timed %d of %d' % (
                   le_best_key, le_best, ex_best, rep['learned_over_exhaustive'],
                   '%.3f' % rep['learned_over_exhaustive_same_run']
                   if rep['learned_over_exhaustive_same_run'] else '-',
-                  le['online_s'], t_train, ex_total, rep['tuning_time_ratio']), flush=True)
+                  le['online_s'], t_train, ex_total, rep['tuning_time_ratio'],
+                  len(le['times']), len(ex_times)), flush=True)
         with open(os.path.join(args.out, 'tuning.json'), 'w') as f:
             json.dump(report, f, indent=1)
     if args.record:
@@ -184,7 +204,7 @@ def record(report, path=None):
         keep[app] = dict((k, r[k]) for k in (
             'exhaustive_best', 'exhaustive_ms', 'learned_best', 'learned_ms',
             'learned_over_exhaustive', 'learned_over_exhaustive_same_run',
-            'tuning_time_ratio', 'tuning_time_ratio_with_training', 'candidates', 'timed'))
+            'tuning_time_ratio', 'timed_ratio', 'candidates', 'timed'))
     with open(path, 'w') as f:
         json.dump(keep, f, indent=1, sort_keys=True)
 

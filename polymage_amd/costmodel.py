@@ -320,6 +320,43 @@ def target_of(rec):
     return math.log(rec['ms'] * 1e6 / (rec['mpix'] * 1e6))
 
 
+def relative_targets(records, clip=1.0):
+    """Per-application targets: log time minus the application's median
+    log time, clipped at ``clip`` (e^1 = 2.7x slower than the median).
+    Only the order of one application's schedules matters, so the
+    application's overall speed is removed, and very slow schedules do not
+    dominate the squared error."""
+    y = np.array([target_of(r) for r in records])
+    apps = [r['app'] for r in records]
+    for a in set(apps):
+        idx = [i for i, x in enumerate(apps) if x == a]
+        y[idx] = np.minimum(y[idx] - np.median(y[idx]), clip)
+    return y
+
+
+class Ensemble(object):
+    """A few MLPs from different initial weights; predictions averaged
+    (the single-network ranking varies a lot with the seed when an
+    application is left out of the training set)."""
+
+    def __init__(self, n=3, epochs=400):
+        self.n = n
+        self.epochs = epochs
+        self.models = []
+
+    def fit(self, X, y):
+        self.models = []
+        losses = []
+        for s in range(self.n):
+            m = MLP(seed=s)
+            losses.append(m.fit(X, y, epochs=self.epochs, lr=3e-3, seed=s))
+            self.models.append(m)
+        return losses
+
+    def predict(self, X):
+        return sum(m.predict(X) for m in self.models) / float(len(self.models))
+
+
 def rank(model, feats_by_key):
     keys = list(feats_by_key)
     X = np.stack([vector(feats_by_key[k]) for k in keys])

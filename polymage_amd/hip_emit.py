@@ -118,10 +118,19 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     w('int pmx_b = pmx_tile;')
     a = plan.anchor
     grid = plan.grid_dims
-    # tile counts (runtime)
+    # tile counts (runtime).  With rolling (pmx_rr > 1) a workgroup takes
+    # pmx_rr consecutive tiles along the first tiled dimension, one per
+    # call (pmx_rk = 0..pmx_rr-1): the halo rows a tile shares with the
+    # previous one, and its own inputs read again by later phases, are
+    # then recent L2 lines.
+    roll_d = next((d for (d, T) in grid if T > 1), None)
     for (d, T) in grid:
         w('const int n%d = ((%s) + %d) / %d;' % (d, _extent_c(a.func, d, base_pr), T - 1, T))
-    ntot = ' * '.join('n%d' % d for (d, T) in grid) or '1'
+    nvar = dict((d, 'n%d' % d) for (d, T) in grid)
+    if roll_d is not None:
+        w('const int nr%d = (n%d + pmx_rr - 1) / pmx_rr;' % (roll_d, roll_d))
+        nvar[roll_d] = 'nr%d' % roll_d
+    ntot = ' * '.join(nvar[d] for (d, T) in grid) or '1'
     w('const int pmx_ntiles = %s;' % ntot)
     if xcd:
         # blocks are dealt round-robin to the 8 XCDs; give each XCD a
@@ -130,7 +139,11 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         w('  const int xcd = pmx_b % 8, k = pmx_b / 8;')
         w('  pmx_b = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k; }')
     for (d, T) in reversed(grid):
-        w('const int t%d = pmx_b %% n%d; pmx_b /= n%d;' % (d, d, d))
+        if d == roll_d:
+            w('const int t%d = (pmx_b %% nr%d) * pmx_rr + pmx_rk; pmx_b /= nr%d;' % (d, d, d))
+            w('if (t%d >= n%d) return;' % (d, d))
+        else:
+            w('const int t%d = pmx_b %% n%d; pmx_b /= n%d;' % (d, d, d))
 
     # extra prologue values: part boxes of every stage
     ir = plan.ir
@@ -616,10 +629,17 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         w('}', 2)
         w('}', 2)
         w('}')
-    grid_c = ' * '.join('((%s + %d) / %d)' % (_extent_c(a.func, d, base_pr), T - 1, T)
+    roll = max(1, int(plan.opts.get('roll', 1))) if roll_d is not None else 1
+    tiles_c = ' * '.join('((%s + %d) / %d)' % (_extent_c(a.func, d, base_pr), T - 1, T)
+                         for (d, T) in grid) or '1'
+    grid_c = ' * '.join(('((((%s + %d) / %d) + %d) / %d)' % (_extent_c(a.func, d, base_pr), T - 1, T,
+                                                              roll - 1, roll))
+                        if d == roll_d else
+                        '((%s + %d) / %d)' % (_extent_c(a.func, d, base_pr), T - 1, T)
                         for (d, T) in grid) or '1'
     info = KernelInfo(kname, 'group', buffers, grid_c, threads, plan.lds_bytes, plan, reads, writes)
-    return wrap_kernel(kname, params, buffers, writes, threads, plan.lds_bytes, L), info
+    info.tiles_c = tiles_c
+    return wrap_kernel(kname, params, buffers, writes, threads, plan.lds_bytes, L, roll=roll), info
 
 
 
@@ -640,7 +660,7 @@ def kernel_args(params, buffers, writes):
     return decls, names, fields
 
 
-def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines):
+def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines, roll=1):
     """The kernel body as a device function plus two entry points:
 
     ``<k>``       one workgroup per tile (grid = number of tiles);
@@ -653,15 +673,23 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines):
     decls, names, fields = kernel_args(params, buffers, writes)
     lds = max(16, lds_bytes)
     L = ['__device__ __forceinline__ void %s_body(%s) {' % (
-        kname, ', '.join(decls + ['const int pmx_tile', 'const int pmx_remap',
-                                  'unsigned char* __restrict__ pmx_lds']))]
-    L += ['  (void)pmx_remap; (void)pmx_lds; (void)pmx_err;']
+        kname, ', '.join(decls + ['const int pmx_tile', 'const int pmx_remap', 'const int pmx_rk',
+                                  'const int pmx_rr', 'unsigned char* __restrict__ pmx_lds']))]
+    L += ['  (void)pmx_remap; (void)pmx_lds; (void)pmx_err; (void)pmx_rk; (void)pmx_rr;']
     L += body_lines
     L.append('}')
     L.append('extern "C" __global__ __launch_bounds__(%d) void %s(%s) {' % (
         threads, kname, ', '.join(decls)))
     L.append('  __shared__ __align__(16) unsigned char pmx_lds[%d];' % lds)
-    L.append('  %s_body(%s);' % (kname, ', '.join(names + ['(int)blockIdx.x', '1', 'pmx_lds'])))
+    if roll > 1:
+        L.append('  for (int pmx_k = 0; pmx_k < %d; ++pmx_k) {' % roll)
+        L.append('    if (pmx_k) __syncthreads();')
+        L.append('    %s_body(%s);' % (kname, ', '.join(names + ['(int)blockIdx.x', '1', 'pmx_k',
+                                                             str(roll), 'pmx_lds'])))
+        L.append('  }')
+    else:
+        L.append('  %s_body(%s);' % (kname, ', '.join(names + ['(int)blockIdx.x', '1', '0', '1',
+                                                             'pmx_lds'])))
     L.append('}')
     L.append('struct %s_frame { %s };' % (kname, ' '.join(fields)))
     L.append('extern "C" __global__ __launch_bounds__(%d) void %s_batch(const %s_frame* __restrict__ F, '
@@ -695,7 +723,8 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines):
     L.append('    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (F[mid].pmx_tile0 <= t) lo = mid; else hi = mid - 1; }')
     L.append('    const %s_frame& f = F[lo];' % kname)
     L.append('    %s_body(%s);' % (kname, ', '.join(['f.' + n for n in names] +
-                                                   ['(int)(t - f.pmx_tile0)', '0', 'pmx_lds'])))
+                                                   ['(int)(t - f.pmx_tile0)', '0', '0', '1',
+                                                    'pmx_lds'])))
     L.append('  }')
     L.append('}')
     return '\n'.join(L) + '\n'

@@ -322,7 +322,7 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
             w('    %s.G_%s = (%s%s*)%s;' % (r, f.name, q, ct, bufname[f]))
         for p in params:
             w('    %s.%s = %s;' % (r, p.name, p.name))
-        w('    %s.pmx_tile0 = (long long)(%s);' % (r, k.grid_c))
+        w('    %s.pmx_tile0 = (long long)(%s);' % (r, getattr(k, 'tiles_c', k.grid_c)))
     w('  }')
     # prefix sums -> first work item of every frame
     for ki, k in enumerate(kernels):

@@ -21,12 +21,12 @@ import kernel_emu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.mark.parametrize("R,C,tiles,block", [(37, 101, [16, 32], None), (62, 70, [32, 64], (2, 4)),
-                                             (9, 13, [4, 8], (4, 4))])
-def test_emulated_harris(R, C, tiles, block):
+@pytest.mark.parametrize("R,C,tiles,opts", [(37, 101, [16, 32], None), (62, 70, [32, 64], {'block': (2, 4)}),
+                                            (9, 13, [4, 8], {'block': (4, 4)}),
+                                            (61, 75, [8, 32], {'roll': 3})])
+def test_emulated_harris(R, C, tiles, opts):
     pd = {}
     out = harris_pipe(pd)
-    opts = {'block': block} if block else None
     pipe = build(out, pd, {'R': R, 'C': C}, 'harris', tiles, gpu_options=opts)
     img = np.random.default_rng(R).random((R + 2, C + 2)).astype(np.float32)
     g = np.full((R + 2, C + 2), -7.0, np.float32)
@@ -37,7 +37,8 @@ def test_emulated_harris(R, C, tiles, block):
 
 
 @pytest.mark.parametrize("opts", [None, {'block': (8, 1)}, {'block': (1, 4)}, {'block': (4, 4)},
-                                  {'stage': 1}, {'stage': 1, 'nt_stores': 1}, {'no_fuse': 1}])
+                                  {'stage': 1}, {'stage': 1, 'nt_stores': 1}, {'no_fuse': 1},
+                                  {'roll': 3}, {'roll': 2, 'block': (2, 1)}])
 def test_emulated_unsharp_options(opts):
     R, C = 45, 77
     pd = {}
