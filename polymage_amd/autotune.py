@@ -11,7 +11,8 @@ For every application:
    compilation), rank them with an ensemble of MLP cost models (HIP
    kernels) trained on the timing records of the *other* applications,
    compile and time the top k, refit with those k measurements, time the
-   next k (two rounds by default), keep the fastest measured;
+   next k, and so on (three rounds of 6 by default), keep the fastest
+   measured;
 3. report, per application: the exhaustive winner, the learned winner, the
    learned winner's time *in the exhaustive run* over the exhaustive best
    (regret free of re-measurement noise) and as re-measured, and the
@@ -39,11 +40,12 @@ def _workload(app):
     return getattr(workloads, app)
 
 
-def exhaustive(app, procs, db, runs=9):
+def exhaustive(app, procs, db, runs=9, pool=None):
     from . import tuner
     cands = list(tuner.default_space(app))
+    tuner.workload_data(_workload(app))
     t0 = time.time()
-    comp = tuner.compile_all(_workload(app), cands, procs=procs, verbose=True)
+    comp = tuner.compile_all(_workload(app), cands, procs=procs, verbose=True, pool=pool)
     t_compile = time.time() - t0
     t1 = time.time()
     times = tuner.time_candidates(_workload(app), comp, runs=runs, db=db, app=app, verbose=True)
@@ -54,7 +56,7 @@ def exhaustive(app, procs, db, runs=9):
             'compile_s': t_compile, 'time_s': t_time, 'n': len(cands)}
 
 
-def learned(app, procs, train_records, k=6, rounds=2, runs=9, dup=20):
+def learned(app, procs, train_records, k=6, rounds=3, runs=9, dup=20, pool=None):
     """The learned search: rank every candidate by the cost model trained on
     the other applications' timings, compile and time the top k, refit with
     those k measurements of this application (weighted by repetition), time
@@ -63,7 +65,7 @@ def learned(app, procs, train_records, k=6, rounds=2, runs=9, dup=20):
     from . import tuner, costmodel
     t0 = time.time()
     cands = list(tuner.default_space(app))
-    feats = tuner.features_all(_workload(app), cands, procs=procs)
+    feats = tuner.features_all(_workload(app), cands, procs=procs, pool=pool)
     feats = dict((key, f) for key, f in feats.items() if f is not None)
     t_feat = time.time() - t0
     keys = list(feats)
@@ -73,6 +75,7 @@ def learned(app, procs, train_records, k=6, rounds=2, runs=9, dup=20):
     times = {}
     t_train = 0.0
     t_top = 0.0
+    t_gpu = 0.0
     tops = []
     for rnd in range(rounds):
         X, y = X0, y0
@@ -92,12 +95,14 @@ def learned(app, procs, train_records, k=6, rounds=2, runs=9, dup=20):
         tops.append(top_keys)
         top = [c for c in cands if c.key in top_keys]
         t2 = time.time()
-        comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True)
+        comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True, pool=pool)
+        t3 = time.time()
         times.update(tuner.time_candidates(_workload(app), comp, runs=runs, db=None, app=app,
                                            verbose=True))
+        t_gpu += time.time() - t3
         t_top += time.time() - t2
     return {'times': times, 'top': [kk for t in tops for kk in t], 'rounds': tops,
-            'features_s': t_feat, 'train_s': t_train, 'top_s': t_top,
+            'features_s': t_feat, 'train_s': t_train, 'top_s': t_top, 'timing_s': t_gpu,
             'online_s': time.time() - t0, 'n_valid': len(feats)}
 
 
@@ -114,7 +119,7 @@ def main():
     ap.add_argument('--out', default='gpurun_out/tune')
     ap.add_argument('--procs', type=int, default=12)
     ap.add_argument('--k', type=int, default=6)
-    ap.add_argument('--rounds', type=int, default=2)
+    ap.add_argument('--rounds', type=int, default=3)
     ap.add_argument('--runs', type=int, default=9)
     ap.add_argument('--reuse-db', default=None, help='timings.jsonl from an earlier run')
     ap.add_argument('--record', action='store_true', help='write polymage_amd/tuned.json')
@@ -127,10 +132,12 @@ def main():
     runtime.CACHE = os.environ['PMX_KCACHE']
     db = TimingDB(os.path.join(args.out, 'timings.jsonl'))
     apps = args.apps.split(',')
+    from .tuner import make_pool
+    pool = make_pool(args.procs)
     ex = {}
     for app in apps:
         print('exhaustive', app, flush=True)
-        ex[app] = exhaustive(app, args.procs, db, runs=args.runs)
+        ex[app] = exhaustive(app, args.procs, db, runs=args.runs, pool=pool)
         best = min(ex[app]['times'].items(), key=lambda kv: kv[1])
         print('  %d candidates (%d timed), best %s %.4f ms, compile %.1fs, timing %.1fs' % (
             ex[app]['n'], len(ex[app]['times']), best[0], best[1], ex[app]['compile_s'],
@@ -145,7 +152,8 @@ def main():
         print('learned', app, 'trained on', len(train), 'records', flush=True)
         # fresh cache again so the learned path pays its own compiles
         runtime.CACHE = tempfile.mkdtemp(prefix='pmx_kc_')
-        le = learned(app, args.procs, train, k=args.k, rounds=args.rounds, runs=args.runs)
+        le = learned(app, args.procs, train, k=args.k, rounds=args.rounds, runs=args.runs,
+                     pool=pool)
         t_train = le['train_s']
         w = _workload(app)(with_data=False)
         ex_times = ex[app]['times']
@@ -171,6 +179,7 @@ def main():
             'learned_s': le['online_s'], 'learned_train_s': t_train,
             'tuning_time_ratio': le['online_s'] / ex_total,
             'timed_ratio': len(le['times']) / float(max(1, len(ex_times))),
+            'gpu_time_ratio': le['timing_s'] / max(1e-9, ex[app]['time_s']),
             'learned_detail': {'features_s': le['features_s'], 'train_s': le['train_s'],
                                'top_s': le['top_s'], 'n_valid': le['n_valid'],
                                'rounds': le['rounds']},
@@ -188,6 +197,8 @@ def main():
                   len(le['times']), len(ex_times)), flush=True)
         with open(os.path.join(args.out, 'tuning.json'), 'w') as f:
             json.dump(report, f, indent=1)
+    pool.close()
+    pool.join()
     if args.record:
         record(report)
 
@@ -204,7 +215,7 @@ def record(report, path=None):
         keep[app] = dict((k, r[k]) for k in (
             'exhaustive_best', 'exhaustive_ms', 'learned_best', 'learned_ms',
             'learned_over_exhaustive', 'learned_over_exhaustive_same_run',
-            'tuning_time_ratio', 'timed_ratio', 'candidates', 'timed'))
+            'tuning_time_ratio', 'timed_ratio', 'gpu_time_ratio', 'candidates', 'timed'))
     with open(path, 'w') as f:
         json.dump(keep, f, indent=1, sort_keys=True)
 

@@ -35,7 +35,8 @@ FEATURES = [
     'occupancy', 'log_threads', 'ops_per_px', 'arith_intensity',
     'hbm_bytes_per_px', 'interm_bytes_per_px', 'idle_lane_frac', 'barriers',
     'log_mpix', 'inline_frac', 'reduction_kernels', 'log_est_ns_px',
-    'waves_per_cu',
+    'waves_per_cu', 'block_pts', 'log_items_per_thread', 'tail_waste', 'tile_aspect',
+    'lds_per_thread',
 ]
 
 # analytic time estimate: vector ALU lanes per second and HBM bytes per
@@ -106,6 +107,9 @@ def features(pipe):
     inline_n = 0
     red = 0
     waves_cu = []
+    blk_w = blk_n = 0.0
+    ipt = []
+    tail = []
     threads = int(pipe.gpu_options.get('threads', 256))
     for p in plans:
         if isinstance(p, GroupPlan):
@@ -134,6 +138,9 @@ def features(pipe):
                 items = int(math.ceil(v / float(U * V)))
                 waves = int(math.ceil(items / float(threads)))
                 idle.append(1.0 - items / float(waves * threads))
+                blk_w += v * U * V
+                blk_n += v
+                ipt.append(math.log2(max(1.0, items / float(threads))))
             anchor_vol = 1
             for e in p.anchor.static_ext:
                 anchor_vol *= e
@@ -141,6 +148,9 @@ def features(pipe):
             lds = max(lds, p.lds_bytes)
             occ.append(min(8, (160 * 1024) // max(1, p.lds_bytes)) if p.lds_bytes else 8)
             waves_cu.append(min(32, occ[-1] * threads // 64) * min(1.0, p.total_tiles / (256.0 * occ[-1])))
+            slots = 256.0 * max(1, min(occ[-1], 2048 // threads))
+            rounds = math.ceil(p.total_tiles / slots)
+            tail.append(1.0 - p.total_tiles / (rounds * slots))
             barriers = max(barriers, p.nphases - 1)
             for s in p.stages:
                 if s.liveout and s.func not in outs:
@@ -189,6 +199,11 @@ def features(pipe):
         'reduction_kernels': red,
         'log_est_ns_px': math.log(est_ns / max(1, out_px)),
         'waves_per_cu': float(min(waves_cu)) if waves_cu else 32.0,
+        'block_pts': blk_w / blk_n if blk_n else 1.0,
+        'log_items_per_thread': float(np.mean(ipt)) if ipt else 0.0,
+        'tail_waste': float(np.mean(tail)) if tail else 0.0,
+        'tile_aspect': math.log2(tiles[0] / float(tiles[1])),
+        'lds_per_thread': lds / float(threads),
     }
     return f
 

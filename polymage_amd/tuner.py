@@ -253,8 +253,17 @@ def _worker_init(cache):
     runtime.CACHE = cache
 
 
+def make_pool(procs):
+    """A pool of fresh worker interpreters ('spawn': they never touch the
+    GPU, and the parent may already have initialised it), reusable across
+    compile_all / features_all calls."""
+    import multiprocessing as mp
+    return mp.get_context('spawn').Pool(procs, initializer=_worker_init, initargs=(runtime.CACHE,))
+
+
 def _features_one(job):
-    modname, fname, kw, cand_tuple = job
+    modname, fname, kw, cand_tuple, cache = job
+    runtime.CACHE = cache
     import importlib
     mod = importlib.import_module(modname)
     fn = getattr(mod, fname)
@@ -267,29 +276,27 @@ def _features_one(job):
         return cand.key, None, str(e)[:300]
 
 
-def features_all(workload, candidates, procs=8, **kw):
+def features_all(workload, candidates, procs=8, pool=None, **kw):
     """Static features of every candidate (planning only, no compilation),
     in parallel worker processes; {key: features or None}."""
-    import multiprocessing as mp
     jobs = [(workload.__module__, workload.__name__, kw,
-             (c.tiles, c.group_size, c.threads)) for c in candidates]
+             (c.tiles, c.group_size, c.threads), runtime.CACHE) for c in candidates]
     out = {}
-    if procs <= 1:
-        res = map(_features_one, jobs)
-    else:
-        ctx = mp.get_context('spawn')
-        pool = ctx.Pool(procs, initializer=_worker_init, initargs=(runtime.CACHE,))
-        res = pool.imap_unordered(_features_one, jobs, chunksize=4)
+    own = pool is None and procs > 1
+    if own:
+        pool = make_pool(procs)
+    res = pool.imap_unordered(_features_one, jobs, chunksize=4) if pool else map(_features_one, jobs)
     for key, feats, err in res:
         out[key] = feats
-    if procs > 1:
+    if own:
         pool.close()
         pool.join()
     return out
 
 
 def _compile_one(job):
-    modname, fname, kw, cand_tuple = job
+    modname, fname, kw, cand_tuple, cache = job
+    runtime.CACHE = cache
     import importlib
     mod = importlib.import_module(modname)
     fn = getattr(mod, fname)
@@ -305,27 +312,22 @@ def _compile_one(job):
         return cand.key, None, None, time.time() - t0, str(e)[:300]
 
 
-def compile_all(workload, candidates, procs=8, verbose=False, **kw):
+def compile_all(workload, candidates, procs=8, verbose=False, pool=None, **kw):
     """Compile candidates in parallel; returns {key: (so_path, features,
-    seconds, error)}.  ``workload`` is a function of apps.workloads.
-    Workers are fresh interpreters ('spawn'): they never touch the GPU,
-    and the parent may already have initialised it."""
-    import multiprocessing as mp
+    seconds, error)}.  ``workload`` is a function of apps.workloads."""
     jobs = [(workload.__module__, workload.__name__, kw,
-             (c.tiles, c.group_size, c.threads)) for c in candidates]
+             (c.tiles, c.group_size, c.threads), runtime.CACHE) for c in candidates]
     out = {}
-    if procs <= 1:
-        res = map(_compile_one, jobs)
-    else:
-        ctx = mp.get_context('spawn')
-        pool = ctx.Pool(procs, initializer=_worker_init, initargs=(runtime.CACHE,))
-        res = pool.imap_unordered(_compile_one, jobs)
+    own = pool is None and procs > 1
+    if own:
+        pool = make_pool(procs)
+    res = pool.imap_unordered(_compile_one, jobs) if pool else map(_compile_one, jobs)
     for key, so, feats, dt, err in res:
         out[key] = (so, feats, dt, err)
         if verbose:
             print('  compiled %-22s %6.1fs%s' % (key, dt, ('  ERROR ' + err[:80]) if err else ''),
                   flush=True)
-    if procs > 1:
+    if own:
         pool.close()
         pool.join()
     return out
@@ -390,9 +392,21 @@ def time_pipeline(bp, w, runs=9, warmup=2, nsets=None):
     return ts[len(ts) // 2]
 
 
+_DATA = {}
+
+
+def workload_data(workload, **kw):
+    """The workload with its synthetic input, generated once per process
+    (input generation is not part of either search's cost)."""
+    dk = (workload.__module__, workload.__name__, repr(sorted(kw.items())))
+    if dk not in _DATA:
+        _DATA[dk] = workload(with_data=True, **kw)
+    return _DATA[dk]
+
+
 def time_candidates(workload, compiled, runs=9, db=None, app=None, verbose=False, **kw):
     """Time compiled candidates on the GPU; returns {key: ms}."""
-    w = workload(with_data=True, **kw)
+    w = workload_data(workload, **kw)
     out = {}
     for key, (so, feats, dt, err) in compiled.items():
         if so is None:
