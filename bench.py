@@ -39,7 +39,7 @@ def parse():
     ap.add_argument('--schedule', default='learned', choices=['learned', 'exhaustive', 'default'],
                     help='schedule recorded by the autotuner for this workload')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
-    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--traffic', default=os.path.join(ROOT, 'profiles', 'traffic.json'),
                     help='JSON file with the HBM bytes per dispatch measured by rocprofv3 PMC runs')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
@@ -77,7 +77,7 @@ def cpu_baseline(w, budget_s):
     best = None
     runs = 0
     t_start = time.perf_counter()
-    while runs < 10 and time.perf_counter() - t_start < budget_s:
+    while runs < 200 and time.perf_counter() - t_start < budget_s:
         t0 = time.perf_counter()
         prog(vals)
         dt = time.perf_counter() - t0
@@ -85,8 +85,9 @@ def cpu_baseline(w, budget_s):
         runs += 1
     return {'value': round(w.mpix / best, 3), 'unit': 'MPix/s', 'cores': int(os.environ['OMP_NUM_THREADS']),
             'kind': 'port',
-            'sample': '%d full-size runs of the oracle C++/OpenMP port of the same pipeline '
-                      '(g++ -O3 -fopenmp -march=native), min time' % runs}
+            'sample': '%d full-size runs (%.1f s of CPU time) of the oracle C++/OpenMP port of the '
+                      'same pipeline (g++ -O3 -fopenmp -march=native), min time' % (
+                          runs, time.perf_counter() - t_start)}
 
 
 def schedule_for(args):
@@ -115,10 +116,12 @@ def batch_bench(args):
     import torch
     from polymage_amd import runtime
     from polymage_amd.apps import workloads, inputs
-    tiles = tuple(int(t) for t in args.tiles.split('x')) if args.tiles else \
-        workloads.DEFAULT_TILES['campipe']
-    gopts = {'threads': args.threads} if args.threads else None
-    w = workloads.campipe(tiles=tiles, gpu_options=gopts, with_data=False)
+    args.workload = 'campipe'
+    sched, sched_info = schedule_for(args)
+    tiles = sched['tiles']
+    gopts = {'threads': sched['threads']} if sched.get('threads') else None
+    kw = {'group_size': sched['group_size']} if sched.get('group_size') else {}
+    w = workloads.campipe(tiles=tiles, gpu_options=gopts, with_data=False, **kw)
     bp = runtime.build(w.pipe)
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
@@ -162,7 +165,7 @@ def batch_bench(args):
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'u16->u8',
         'data': 'synthetic (48 seeded 10-bit GRBG frames, sizes 32*U{16..81} x 32*U{12..61})',
         'config': {'workload': 'campipe x48 (%.1f MPix out)' % mpix, 'tiles': list(tiles),
-                   'kernels': len(bp.code.kernels)},
+                   'kernels': len(bp.code.kernels), 'schedule': sched_info},
         'batch': {'frames_ms': round(res['frames'], 4), 'persistent_ms': round(res['persistent'], 4),
                   'graph_ms': round(res['graph'], 4),
                   'speedup_persistent_vs_frames': round(res['frames'] / res['persistent'], 3),

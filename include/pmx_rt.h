@@ -112,6 +112,13 @@ int pmx_mlp_backward(const float* params, const int* dims, int nlayers,
 int pmx_mlp_adam(float* params, const float* grads, float* m, float* v,
                  int count, float lr, float beta1, float beta2, float eps,
                  int step, void* stream);
+/* `epochs` full-batch epochs (forward, backward, Adam) launched back to
+ * back; loss[e] = loss of epoch e (device array of `epochs` floats); Adam
+ * steps step0+1 .. step0+epochs.  `acts` and `grads` as above. */
+int pmx_mlp_train(float* params, const int* dims, int nlayers, const float* x,
+                  const float* target, int batch, int epochs, float lr, float beta1,
+                  float beta2, float eps, float* m, float* v, float* acts, float* grads,
+                  float* loss, int step0, void* stream);
 
 #ifdef __cplusplus
 }

@@ -309,14 +309,24 @@ class MLP(object):
         self.ymu = float(y.mean())
         xs = torch.from_numpy(self._norm(X).astype(np.float32)).to(self.device)
         ys = torch.from_numpy((y - self.ymu).reshape(-1, 1).astype(np.float32)).to(self.device)
-        losses = []
-        for ep in range(epochs):
-            out, acts = self.forward_raw(xs)
-            grads, loss = self.backward_raw(acts, ys)
-            self.adam(grads, lr)
-            if ep % 50 == 0 or ep == epochs - 1:
-                losses.append(float(loss.item()))
-        return losses
+        # the whole loop in one runtime call (kernels launched back to back)
+        acts = self._acts(xs.shape[0])
+        grads = torch.empty_like(self.params)
+        loss = torch.empty(epochs, dtype=torch.float32, device=self.device)
+        rc = self.rt.pmx_mlp_train(ctypes.c_void_p(self.params.data_ptr()), self.cdims, self.nl,
+                                   ctypes.c_void_p(xs.data_ptr()), ctypes.c_void_p(ys.data_ptr()),
+                                   xs.shape[0], epochs, ctypes.c_float(lr), ctypes.c_float(0.9),
+                                   ctypes.c_float(0.999), ctypes.c_float(1e-8),
+                                   ctypes.c_void_p(self.m.data_ptr()),
+                                   ctypes.c_void_p(self.v.data_ptr()),
+                                   ctypes.c_void_p(acts.data_ptr()),
+                                   ctypes.c_void_p(grads.data_ptr()),
+                                   ctypes.c_void_p(loss.data_ptr()), self.step_n, self._stream())
+        if rc:
+            raise RuntimeError('pmx_mlp_train: %d' % rc)
+        self.step_n += epochs
+        lh = loss.cpu().numpy()
+        return [float(lh[e]) for e in range(epochs) if e % 50 == 0 or e == epochs - 1]
 
     def predict(self, X):
         import torch

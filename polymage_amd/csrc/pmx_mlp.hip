@@ -191,4 +191,26 @@ int pmx_mlp_adam(float* params, const float* grads, float* m, float* v, int coun
   return hip_ok(hipGetLastError(), "mlp: adam launch");
 }
 
+// Full-batch training loop in one call: forward, MSE backward and Adam for
+// `epochs` epochs, launched back to back on `stream` (no host round trip per
+// epoch).  loss[e] receives epoch e's loss; Adam's step count starts at
+// step0 + 1.
+int pmx_mlp_train(float* params, const int* dims, int nlayers, const float* x,
+                  const float* target, int batch, int epochs, float lr, float beta1,
+                  float beta2, float eps, float* m, float* v, float* acts, float* grads,
+                  float* loss, int step0, void* stream) {
+  int count = 0;
+  int rc = pmx_mlp_param_count(dims, nlayers, &count);
+  if (rc) return rc;
+  for (int e = 0; e < epochs; ++e) {
+    rc = pmx_mlp_forward(params, dims, nlayers, x, batch, acts, stream);
+    if (rc) return rc;
+    rc = pmx_mlp_backward(params, dims, nlayers, acts, target, batch, grads, loss + e, stream);
+    if (rc) return rc;
+    rc = pmx_mlp_adam(params, grads, m, v, count, lr, beta1, beta2, eps, step0 + e + 1, stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -660,6 +660,9 @@ def kernel_args(params, buffers, writes):
     return decls, names, fields
 
 
+BATCH_LDS_FRAMES = 256      # frame tables up to this size are staged in LDS
+
+
 def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines, roll=1):
     """The kernel body as a device function plus two entry points:
 
@@ -697,34 +700,44 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines, 
              % (threads, kname, kname))
     L.append('  __shared__ __align__(16) unsigned char pmx_lds[%d];' % lds)
     L.append('  __shared__ long long pmx_work;')
+    L.append('  __shared__ long long pmx_t0[%d];' % BATCH_LDS_FRAMES)
+    L.append('  // first work item of every frame, staged in LDS for the frame lookup')
+    L.append('  const bool pmx_small = nframes <= %d;' % BATCH_LDS_FRAMES)
+    L.append('  if (pmx_small) for (int i = threadIdx.x; i < nframes; i += blockDim.x) pmx_t0[i] = F[i].pmx_tile0;')
     L.append('  // 8 queues, one per XCD (workgroups are dealt round-robin to the')
     L.append('  // XCDs): queue q holds the contiguous items [total*q/8, total*(q+1)/8),')
     L.append('  // so neighbouring tiles share an L2; an empty home queue sends the')
-    L.append('  // workgroup on to the next ones until all 8 are empty.')
+    L.append('  // workgroup on to the next ones until all 8 are empty.  The next')
+    L.append('  // item is fetched while the current one is processed.')
     L.append('  const int home = (int)(blockIdx.x & 7);')
     L.append('  int q = 0;')
-    L.append('  for (;;) {')
-    L.append('    __syncthreads();')
-    L.append('    if (threadIdx.x == 0) {')
-    L.append('      long long w = -1;')
-    L.append('      while (q < 8) {')
-    L.append('        const int qq = (home + q) & 7;')
-    L.append('        const long long lo = total * qq / 8, hi = total * (qq + 1) / 8;')
-    L.append('        const long long k = (long long)atomicAdd(&counter[qq], 1ULL);')
-    L.append('        if (lo + k < hi) { w = lo + k; break; }')
-    L.append('        ++q;')
-    L.append('      }')
-    L.append('      pmx_work = w;')
+    L.append('  auto pmx_fetch = [&]() -> long long {')
+    L.append('    while (q < 8) {')
+    L.append('      const int qq = (home + q) & 7;')
+    L.append('      const long long lo = total * qq / 8, hi = total * (qq + 1) / 8;')
+    L.append('      const long long k = (long long)atomicAdd(&counter[qq], 1ULL);')
+    L.append('      if (lo + k < hi) return lo + k;')
+    L.append('      ++q;')
     L.append('    }')
-    L.append('    __syncthreads();')
+    L.append('    return -1;')
+    L.append('  };')
+    L.append('  if (threadIdx.x == 0) pmx_work = pmx_fetch();')
+    L.append('  __syncthreads();')
+    L.append('  for (;;) {')
     L.append('    const long long t = pmx_work;')
     L.append('    if (t < 0) break;')
+    L.append('    long long nxt = -1;')
+    L.append('    if (threadIdx.x == 0) nxt = pmx_fetch();')
     L.append('    int lo = 0, hi = nframes - 1;')
-    L.append('    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (F[mid].pmx_tile0 <= t) lo = mid; else hi = mid - 1; }')
+    L.append('    if (pmx_small) { while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (pmx_t0[mid] <= t) lo = mid; else hi = mid - 1; } }')
+    L.append('    else { while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (F[mid].pmx_tile0 <= t) lo = mid; else hi = mid - 1; } }')
     L.append('    const %s_frame& f = F[lo];' % kname)
     L.append('    %s_body(%s);' % (kname, ', '.join(['f.' + n for n in names] +
                                                    ['(int)(t - f.pmx_tile0)', '0', '0', '1',
                                                     'pmx_lds'])))
+    L.append('    __syncthreads();')
+    L.append('    if (threadIdx.x == 0) pmx_work = nxt;')
+    L.append('    __syncthreads();')
     L.append('  }')
     L.append('}')
     return '\n'.join(L) + '\n'

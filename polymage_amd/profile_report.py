@@ -56,6 +56,9 @@ def main():
     ap.add_argument('--ideal', type=float, default=None, help='ideal bytes per dispatch')
     ap.add_argument('--ideal-in', type=float, default=None, help='input bytes per dispatch')
     ap.add_argument('--commands', default='', help='text: the commands that made the data')
+    ap.add_argument('--calls', type=int, default=0,
+                    help='pipeline calls in the PMC runs: sum the matching dispatches per call '
+                         '(pipelines of several kernels); 0 = average per dispatch')
     ap.add_argument('--out', required=True)
     ap.add_argument('--traffic-json')
     ap.add_argument('--code-key')
@@ -73,13 +76,18 @@ def main():
             L.append('| %s | %s | %.2f | %.2f | %.2f | %.2f |' % (
                 r['Name'][:60], r['Calls'], float(r['AverageNs']) / 1e3, float(r['MinNs']) / 1e3,
                 float(r['MaxNs']) / 1e3, float(r['Percentage'])))
-            if r['Name'].startswith(args.kernel):
+            if r['Name'] == args.kernel or (not args.calls and r['Name'].startswith(args.kernel)):
                 avg_us = float(r['AverageNs']) / 1e3
         L.append('')
     traffic = None
     if args.fetch and args.write and args.copy_fetch and args.copy_write:
-        f = statistics.mean(counter(args.fetch, args.kernel, 'FETCH_SIZE'))
-        wr = statistics.mean(counter(args.write, args.kernel, 'WRITE_SIZE'))
+        if args.calls:
+            # multi-kernel pipeline: all matching dispatches per pipeline call
+            f = sum(counter(args.fetch, args.kernel, 'FETCH_SIZE')) / args.calls
+            wr = sum(counter(args.write, args.kernel, 'WRITE_SIZE')) / args.calls
+        else:
+            f = statistics.mean(counter(args.fetch, args.kernel, 'FETCH_SIZE'))
+            wr = statistics.mean(counter(args.write, args.kernel, 'WRITE_SIZE'))
         cf = statistics.median(counter(args.copy_fetch, '__amd_rocclr_copyBuffer', 'FETCH_SIZE'))
         cw = statistics.median(counter(args.copy_write, '__amd_rocclr_copyBuffer', 'WRITE_SIZE'))
         kib = args.copy_mib * 1024
@@ -124,6 +132,7 @@ def main():
             with open(args.traffic_json) as fi:
                 rec = json.load(fi)
         rec[args.workload] = {'bytes_per_dispatch': round(traffic), 'code_key': args.code_key,
+                              'per': 'pipeline call' if args.calls else 'dispatch',
                               'kernel': args.kernel, 'summary': os.path.basename(args.out)}
         with open(args.traffic_json, 'w') as fo:
             json.dump(rec, fo, indent=1)

@@ -161,3 +161,4 @@ extern "C" int pmx_mlp_param_count(const int*, int, int*) { return fail("emulato
 extern "C" int pmx_mlp_forward(const float*, const int*, int, const float*, int, float*, void*) { return fail("emulator"); }
 extern "C" int pmx_mlp_backward(const float*, const int*, int, const float*, const float*, int, float*, float*, void*) { return fail("emulator"); }
 extern "C" int pmx_mlp_adam(float*, const float*, float*, float*, int, float, float, float, float, int, void*) { return fail("emulator"); }
+extern "C" int pmx_mlp_train(float*, const int*, int, const float*, const float*, int, int, float, float, float, float, float*, float*, float*, float*, float*, int, void*) { return fail("emulator"); }

@@ -145,3 +145,26 @@ def test_learned_search_gains_over_search_without_the_model(gpu):
         print(app, 'learned %.3f random %.3f analytic %.3f' % (learned[-1], rand[-1], analytic[-1]))
     assert min(learned) >= 0.95, learned
     assert np.mean(learned) > np.mean(rand) and np.mean(learned) > np.mean(analytic)
+
+
+@pytest.mark.gpu
+def test_hip_training_loop_equals_stepwise_training(gpu):
+    """pmx_mlp_train (all epochs in one call) runs exactly the kernels of
+    forward / backward / Adam called one epoch at a time."""
+    skip_if_prebuild()
+    import torch
+    from polymage_amd.costmodel import MLP
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (64, 6))
+    y = X[:, 0] * X[:, 1] - X[:, 2]
+    a = MLP(dims=[6, 16, 16, 1], seed=3)
+    a.fit(X, y, epochs=25, lr=3e-3)
+    b = MLP(dims=[6, 16, 16, 1], seed=3)
+    b.mu, b.sd, b.ymu = X.mean(0).astype(np.float32), X.std(0).astype(np.float32) + 1e-3, float(y.mean())
+    xs = torch.from_numpy(b._norm(X.astype(np.float32)).astype(np.float32)).to(b.device)
+    ys = torch.from_numpy((y.astype(np.float32) - b.ymu).reshape(-1, 1).astype(np.float32)).to(b.device)
+    for _ in range(25):
+        out, acts = b.forward_raw(xs)
+        grads, loss = b.backward_raw(acts, ys)
+        b.adam(grads, 3e-3)
+    assert torch.equal(a.params, b.params)
