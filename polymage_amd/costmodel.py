@@ -364,7 +364,7 @@ class Ensemble(object):
     (the single-network ranking varies a lot with the seed when an
     application is left out of the training set)."""
 
-    def __init__(self, n=3, epochs=400):
+    def __init__(self, n=5, epochs=400):
         self.n = n
         self.epochs = epochs
         self.models = []
