@@ -160,9 +160,12 @@ def test_hip_training_loop_equals_stepwise_training(gpu):
     a = MLP(dims=[6, 16, 16, 1], seed=3)
     a.fit(X, y, epochs=25, lr=3e-3)
     b = MLP(dims=[6, 16, 16, 1], seed=3)
-    b.mu, b.sd, b.ymu = X.mean(0).astype(np.float32), X.std(0).astype(np.float32) + 1e-3, float(y.mean())
-    xs = torch.from_numpy(b._norm(X.astype(np.float32)).astype(np.float32)).to(b.device)
-    ys = torch.from_numpy((y.astype(np.float32) - b.ymu).reshape(-1, 1).astype(np.float32)).to(b.device)
+    # the same normalisation fit() applies
+    b.mu, b.sd, b.ymu = a.mu, a.sd, a.ymu
+    Xf = np.asarray(X, np.float32)
+    yf = np.asarray(y, np.float32)
+    xs = torch.from_numpy(b._norm(Xf).astype(np.float32)).to(b.device)
+    ys = torch.from_numpy((yf - b.ymu).reshape(-1, 1).astype(np.float32)).to(b.device)
     for _ in range(25):
         out, acts = b.forward_raw(xs)
         grads, loss = b.backward_raw(acts, ys)
