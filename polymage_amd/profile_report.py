@@ -72,12 +72,20 @@ def main():
         rows = _csv(args.trace, '_kernel_stats.csv')
         L += ['Kernel statistics (`--kernel-trace --stats`):', '',
               '| kernel | calls | average (us) | min (us) | max (us) | % |', '|---|---|---|---|---|---|']
+        per_call = 0.0
         for r in rows:
             L.append('| %s | %s | %.2f | %.2f | %.2f | %.2f |' % (
                 r['Name'][:60], r['Calls'], float(r['AverageNs']) / 1e3, float(r['MinNs']) / 1e3,
                 float(r['MaxNs']) / 1e3, float(r['Percentage'])))
             if r['Name'] == args.kernel or (not args.calls and r['Name'].startswith(args.kernel)):
                 avg_us = float(r['AverageNs']) / 1e3
+            elif args.calls and r['Name'].startswith(args.kernel):
+                # every kernel of the pipeline runs once per call
+                per_call += float(r['AverageNs']) / 1e3
+        if avg_us is None and per_call:
+            avg_us = per_call
+            L += ['', 'Sum of the average durations of the `%s*` kernels (one pipeline call): '
+                      '%.2f us.' % (args.kernel, per_call)]
         L.append('')
     traffic = None
     if args.fetch and args.write and args.copy_fetch and args.copy_write:
