@@ -134,6 +134,11 @@ class Printer(object):
 # Device helpers that pin the overloads libstdc++ picks on the host.
 DEVICE_PRELUDE = r'''
 #define PMX_BIG 536870912
+#ifndef PMX_GLOBAL
+#define PMX_GLOBAL __attribute__((address_space(1)))
+#define PMX_CONST __attribute__((address_space(4)))
+#define PMX_OPAQUE(x) __asm__ volatile("" : "+v"(x))
+#endif
 __device__ __forceinline__ int pmx_abs(int v) { return v < 0 ? -v : v; }
 __device__ __forceinline__ long long pmx_abs(long long v) { return v < 0 ? -v : v; }
 __device__ __forceinline__ float pmx_abs(float v) { return __builtin_fabsf(v); }

@@ -32,7 +32,7 @@ import math
 
 from .analysis import (PlanError, classify_access, case_parts, ceval_int,
                        eval_bexpr, shift_access)
-from .constructs import Reference, Reduction
+from .constructs import Reference, Reduction, Variable
 from .region_ir import RegionIR, BIG
 
 LDS_LIMIT_BYTES = 160 * 1024
@@ -371,6 +371,32 @@ class GroupPlan(object):
                 # (never the innermost dimension: splitting it would leave
                 # each workgroup a strided, uncoalesced slice of the rows)
                 self.grid_dims.append((d, 1))
+        # split dimensions whose workgroups read the same input tile (the
+        # local Laplacian's K levels all read the gray image) vary fastest
+        # in the workgroup order, so those workgroups run together on one
+        # XCD and the shared tile is read from HBM once, not once per level
+        self.split_fast = [d for (d, T) in self.grid_dims
+                           if T == 1 and self._split_shares_inputs(d)]
+
+    def _split_shares_inputs(self, d):
+        """True when some stage reads a buffer from outside the group at an
+        index that does not involve its dimension aligned with anchor dim
+        d: every workgroup of a split along d then reads the same points."""
+        for s in self.stages:
+            own = [s.func.variables[i] for i in range(s.nd)
+                   if s.align[i] is not None and s.align[i][0] == d]
+            if not own:
+                continue
+            for part in s.parts:
+                for r in self._part_refs(part):
+                    if self.in_group(r.objectRef):
+                        continue
+                    used = set()
+                    for arg in r.arguments:
+                        used.update(arg.collect(Variable) if hasattr(arg, 'collect') else [])
+                    if not any(v in used for v in own):
+                        return True
+        return False
 
     def _identity_dim(self, d):
         """True when every in-group dependence along anchor dim d is the

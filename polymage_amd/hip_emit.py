@@ -138,7 +138,8 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         w('if (pmx_remap) { const int per = pmx_ntiles / 8, rem = pmx_ntiles % 8;')
         w('  const int xcd = pmx_b % 8, k = pmx_b / 8;')
         w('  pmx_b = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k; }')
-    for (d, T) in reversed(grid):
+    fast = [g for g in grid if g[0] in getattr(plan, 'split_fast', ())]
+    for (d, T) in fast + [g for g in reversed(grid) if g not in fast]:
         if d == roll_d:
             w('const int t%d = (pmx_b %% nr%d) * pmx_rr + pmx_rk; pmx_b /= nr%d;' % (d, d, d))
             w('if (t%d >= n%d) return;' % (d, d))
@@ -616,7 +617,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             continue
         total_items = sum(n for (_, n, _) in sws)
         w('{ // phase %d: %s' % (ph, '; '.join(c for (c, _, _) in sws)))
-        w('for (int pmx_it = threadIdx.x; pmx_it < %d; pmx_it += %d) {' % (total_items, threads), 2)
+        w('for (int pmx_it = pmx_tx; pmx_it < %d; pmx_it += %d) {' % (total_items, threads), 2)
         base = 0
         for k, (c, n, SL) in enumerate(sws):
             if len(sws) == 1:
@@ -660,9 +661,6 @@ def kernel_args(params, buffers, writes):
     return decls, names, fields
 
 
-BATCH_LDS_FRAMES = 256      # frame tables up to this size are staged in LDS
-
-
 def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines, roll=1):
     """The kernel body as a device function plus two entry points:
 
@@ -675,10 +673,33 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines, 
     counters have passed their queue's end, so the grid always drains."""
     decls, names, fields = kernel_args(params, buffers, writes)
     lds = max(16, lds_bytes)
-    L = ['__device__ __forceinline__ void %s_body(%s) {' % (
-        kname, ', '.join(decls + ['const int pmx_tile', 'const int pmx_remap', 'const int pmx_rk',
-                                  'const int pmx_rr', 'unsigned char* __restrict__ pmx_lds']))]
-    L += ['  (void)pmx_remap; (void)pmx_lds; (void)pmx_err; (void)pmx_rk; (void)pmx_rr;']
+    tail = ['const int pmx_tile', 'const int pmx_remap', 'const int pmx_rk', 'const int pmx_rr',
+            'unsigned char* __restrict__ pmx_lds']
+    prologue = '  (void)pmx_remap; (void)pmx_lds; (void)pmx_err; (void)pmx_rk; (void)pmx_rr;'
+    # plain form: kernel arguments, whose address space and aliasing the
+    # compiler knows, and the thread index with its known range
+    L = ['__device__ __forceinline__ void %s_body(%s) {' % (kname, ', '.join(decls + tail)),
+         prologue, '  const int pmx_tx = (int)threadIdx.x; (void)pmx_tx;']
+    L += body_lines
+    L.append('}')
+    # persistent form: the same body over pointers read from frame records.
+    # Buffers the kernel writes are global-address-space pointers (global,
+    # not flat, stores); buffers it only reads are constant-address-space
+    # pointers (they do not change during the launch, so reads at uniform
+    # addresses become scalar loads).  The thread index is opaque to the
+    # optimiser, so per-thread index arithmetic is recomputed for every work
+    # item instead of being hoisted out of the work loop into registers.
+    ptypes = {'pmx_err': 'int'}
+    for f in buffers:
+        ptypes['G_' + f.name] = ('' if f in writes else 'const ') + c_type(f.typ)
+    aspace = dict((n, 'PMX_CONST' if t.startswith('const ') else 'PMX_GLOBAL')
+                  for n, t in ptypes.items())
+    bdecls = ['%s %s* __restrict__ %s' % (ptypes[n], aspace[n], n) if n in ptypes else d
+              for n, d in zip(names, decls)]
+    fargs = ['(%s %s*)f.%s' % (ptypes[n], aspace[n], n) if n in ptypes else 'f.' + n
+             for n in names]
+    L += ['__device__ __forceinline__ void %s_bbody(%s) {' % (kname, ', '.join(bdecls + tail)),
+          prologue, '  int pmx_tx = (int)threadIdx.x; PMX_OPAQUE(pmx_tx); (void)pmx_tx;']
     L += body_lines
     L.append('}')
     L.append('extern "C" __global__ __launch_bounds__(%d) void %s(%s) {' % (
@@ -700,10 +721,9 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines, 
              % (threads, kname, kname))
     L.append('  __shared__ __align__(16) unsigned char pmx_lds[%d];' % lds)
     L.append('  __shared__ long long pmx_work;')
-    L.append('  __shared__ long long pmx_t0[%d];' % BATCH_LDS_FRAMES)
-    L.append('  // first work item of every frame, staged in LDS for the frame lookup')
-    L.append('  const bool pmx_small = nframes <= %d;' % BATCH_LDS_FRAMES)
-    L.append('  if (pmx_small) for (int i = threadIdx.x; i < nframes; i += blockDim.x) pmx_t0[i] = F[i].pmx_tile0;')
+    L.append('  // frame records do not change during the launch: read through the')
+    L.append('  // constant address space they are scalar loads')
+    L.append('  const %s_frame PMX_CONST* __restrict__ Fc = (const %s_frame PMX_CONST*)F;' % (kname, kname))
     L.append('  // 8 queues, one per XCD (workgroups are dealt round-robin to the')
     L.append('  // XCDs): queue q holds the contiguous items [total*q/8, total*(q+1)/8),')
     L.append('  // so neighbouring tiles share an L2; an empty home queue sends the')
@@ -724,15 +744,25 @@ def wrap_kernel(kname, params, buffers, writes, threads, lds_bytes, body_lines, 
     L.append('  if (threadIdx.x == 0) pmx_work = pmx_fetch();')
     L.append('  __syncthreads();')
     L.append('  for (;;) {')
-    L.append('    const long long t = pmx_work;')
+    L.append('    // the work item and its frame are the same for the whole workgroup:')
+    L.append('    // readfirstlane makes that visible, so the frame record (copied')
+    L.append('    // word by word) and all per-tile index arithmetic stay in scalar')
+    L.append('    // registers')
+    L.append('    const long long tw = pmx_work;')
+    L.append('    const long long t = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane('
+             '(int)(tw >> 32)) << 32) | (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)tw));')
     L.append('    if (t < 0) break;')
     L.append('    long long nxt = -1;')
     L.append('    if (threadIdx.x == 0) nxt = pmx_fetch();')
+    L.append('    // the frame holding work item t: last frame whose first item is <= t')
     L.append('    int lo = 0, hi = nframes - 1;')
-    L.append('    if (pmx_small) { while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (pmx_t0[mid] <= t) lo = mid; else hi = mid - 1; } }')
-    L.append('    else { while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (F[mid].pmx_tile0 <= t) lo = mid; else hi = mid - 1; } }')
-    L.append('    const %s_frame& f = F[lo];' % kname)
-    L.append('    %s_body(%s);' % (kname, ', '.join(['f.' + n for n in names] +
+    L.append('    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (Fc[mid].pmx_tile0 <= t) lo = mid; else hi = mid - 1; }')
+    L.append('    lo = __builtin_amdgcn_readfirstlane(lo);')
+    L.append('    %s_frame f;' % kname)
+    L.append('    { int w[sizeof(f) / 4]; __builtin_memcpy(w, (const void*)&Fc[lo], sizeof(f));')
+    L.append('      for (int i = 0; i < (int)(sizeof(f) / 4); ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);')
+    L.append('      __builtin_memcpy(&f, w, sizeof(f)); }')
+    L.append('    %s_bbody(%s);' % (kname, ', '.join(fargs +
                                                    ['(int)(t - f.pmx_tile0)', '0', '0', '1',
                                                     'pmx_lds'])))
     L.append('    __syncthreads();')

@@ -62,6 +62,8 @@ def main():
     ap.add_argument('--out', required=True)
     ap.add_argument('--traffic-json')
     ap.add_argument('--code-key')
+    ap.add_argument('--traffic-key', default=None,
+                    help='record name in --traffic-json (default: --workload)')
     args = ap.parse_args()
 
     L = ['# %s — %s' % (args.kernel, args.workload), '']
@@ -121,8 +123,9 @@ def main():
                 avg_us, (args.ideal or 0) / avg_us / 1e3, traffic / avg_us / 1e3))
         L.append('')
     if args.sq:
-        names = ['SQ_WAVES', 'SQ_INSTS_VALU', 'SQ_INSTS_VMEM_RD', 'SQ_INSTS_VMEM_WR', 'SQ_INSTS_LDS',
-                 'SQ_LDS_BANK_CONFLICT', 'SQ_ACTIVE_INST_VALU', 'SQ_BUSY_CYCLES', 'GRBM_GUI_ACTIVE']
+        names = ['SQ_WAVES', 'SQ_INSTS_VALU', 'SQ_INSTS_SALU', 'SQ_INSTS_SMEM', 'SQ_INSTS_VMEM_RD',
+                 'SQ_INSTS_VMEM_WR', 'SQ_INSTS_LDS', 'SQ_LDS_BANK_CONFLICT', 'SQ_ACTIVE_INST_VALU',
+                 'SQ_BUSY_CYCLES', 'SQ_WAVE_CYCLES', 'GRBM_GUI_ACTIVE']
         L += ['SQ counters per dispatch (one `--pmc` run):', '', '| counter | value |', '|---|---|']
         for n in names:
             try:
@@ -139,7 +142,7 @@ def main():
         if os.path.exists(args.traffic_json):
             with open(args.traffic_json) as fi:
                 rec = json.load(fi)
-        rec[args.workload] = {'bytes_per_dispatch': round(traffic), 'code_key': args.code_key,
+        rec[args.traffic_key or args.workload] = {'bytes_per_dispatch': round(traffic), 'code_key': args.code_key,
                               'per': 'pipeline call' if args.calls else 'dispatch',
                               'kernel': args.kernel, 'summary': os.path.basename(args.out)}
         with open(args.traffic_json, 'w') as fo:

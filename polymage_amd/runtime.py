@@ -173,6 +173,23 @@ class BuiltPipeline(object):
         args.append(ctypes.c_void_p(stream))
         check(self.fn_dev(*args), 'pipeline_%s_dev' % self.name)
 
+    def bind_dev(self, values, stream=0):
+        """A zero-argument callable that enqueues ``run_dev(values, stream)``
+        with the ctypes arguments marshalled once (for launch loops whose
+        kernels are shorter than Python's per-call marshalling)."""
+        args = self._param_args(values)
+        for f in self.inputs + self.outputs:
+            v = values[f.name]
+            args.append(ctypes.c_void_p(v.data_ptr() if hasattr(v, 'data_ptr') else int(v)))
+        args.append(ctypes.c_void_p(stream))
+        fn, name = self.fn_dev, 'pipeline_%s_dev' % self.name
+
+        def launch():
+            rc = fn(*args)
+            if rc:
+                check(rc, name)
+        return launch
+
 
 def load(so_path, pipe):
     """Load an already compiled pipeline library for ``pipe``'s signature."""

@@ -19,6 +19,10 @@ extern pmx_dim3 gridDim;
 #define __align__(n) __attribute__((aligned(n)))
 void __syncthreads();
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
+#define __builtin_amdgcn_readfirstlane(v) (v)
+#define PMX_GLOBAL
+#define PMX_CONST
+#define PMX_OPAQUE(x)
 static inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 static inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 static inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
