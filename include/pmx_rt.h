@@ -119,6 +119,16 @@ int pmx_mlp_train(float* params, const int* dims, int nlayers, const float* x,
                   const float* target, int batch, int epochs, float lr, float beta1,
                   float beta2, float eps, float* m, float* v, float* acts, float* grads,
                   float* loss, int step0, void* stream);
+/* The same loop for `nmodels` networks of one shape trained side by side
+ * on the same x / target (an ensemble from different initial weights):
+ * params, m, v and grads are [nmodels][count]; `hidden` holds each model's
+ * activations of layers 1..nlayers ([nmodels][sum_l batch*dims[l]]; x is
+ * read in place as layer 0); loss is [nmodels][epochs].  Each model's
+ * arithmetic is exactly that of pmx_mlp_train on its own. */
+int pmx_mlp_train_ensemble(float* params, const int* dims, int nlayers, int nmodels,
+                           const float* x, const float* target, int batch, int epochs,
+                           float lr, float beta1, float beta2, float eps, float* m, float* v,
+                           float* hidden, float* grads, float* loss, int step0, void* stream);
 
 #ifdef __cplusplus
 }

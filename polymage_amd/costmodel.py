@@ -12,7 +12,8 @@ GPU): per pipeline, aggregated over its kernels,
 The model is a small fully connected ReLU network (``dims`` below) whose
 forward pass, mean-squared-error backward pass and Adam update run as the
 hand-written HIP kernels of libpmx_rt (``pmx_mlp_forward`` /
-``pmx_mlp_backward`` / ``pmx_mlp_adam``).  It regresses log(ns per output
+``pmx_mlp_backward`` / ``pmx_mlp_adam``; the training loop of a whole
+ensemble is one ``pmx_mlp_train_ensemble`` call).  It regresses log(ns per output
 pixel) from standardised features.  Candidates are ranked by predicted
 time; the exhaustive tuner remains the fallback and the judge of regret.
 """
@@ -369,14 +370,44 @@ class Ensemble(object):
         self.epochs = epochs
         self.models = []
 
-    def fit(self, X, y):
-        self.models = []
-        losses = []
-        for s in range(self.n):
-            m = MLP(seed=s)
-            losses.append(m.fit(X, y, epochs=self.epochs, lr=3e-3, seed=s))
-            self.models.append(m)
-        return losses
+    def fit(self, X, y, lr=3e-3):
+        """Train all members in one runtime call (``pmx_mlp_train_ensemble``:
+        every kernel covers all members, blockIdx.y = member); each member
+        ends exactly as ``MLP(seed=s).fit(X, y, epochs, lr)`` would."""
+        import torch
+        members = [MLP(seed=s) for s in range(self.n)]
+        m0 = members[0]
+        X = np.asarray(X, np.float32)
+        y = np.asarray(y, np.float32)
+        mu, sd, ymu = X.mean(0), X.std(0) + 1e-3, float(y.mean())
+        for m in members:
+            m.mu, m.sd, m.ymu = mu, sd, ymu
+        xs = torch.from_numpy(m0._norm(X).astype(np.float32)).to(m0.device)
+        ys = torch.from_numpy((y - ymu).reshape(-1, 1).astype(np.float32)).to(m0.device)
+        batch = xs.shape[0]
+        params = torch.stack([m.params for m in members]).contiguous()
+        mm = torch.zeros_like(params)
+        vv = torch.zeros_like(params)
+        grads = torch.empty_like(params)
+        hidden = torch.empty(self.n * sum(batch * d for d in m0.dims[1:]), dtype=torch.float32,
+                             device=m0.device)
+        loss = torch.empty((self.n, self.epochs), dtype=torch.float32, device=m0.device)
+        rc = m0.rt.pmx_mlp_train_ensemble(
+            ctypes.c_void_p(params.data_ptr()), m0.cdims, m0.nl, self.n,
+            ctypes.c_void_p(xs.data_ptr()), ctypes.c_void_p(ys.data_ptr()), batch, self.epochs,
+            ctypes.c_float(lr), ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8),
+            ctypes.c_void_p(mm.data_ptr()), ctypes.c_void_p(vv.data_ptr()),
+            ctypes.c_void_p(hidden.data_ptr()), ctypes.c_void_p(grads.data_ptr()),
+            ctypes.c_void_p(loss.data_ptr()), 0, m0._stream())
+        if rc:
+            raise RuntimeError('pmx_mlp_train_ensemble: %d' % rc)
+        for i, m in enumerate(members):
+            m.params, m.m, m.v = params[i], mm[i], vv[i]
+            m.step_n = self.epochs
+        self.models = members
+        lh = loss.cpu().numpy()
+        return [[float(lh[i, e]) for e in range(self.epochs) if e % 50 == 0 or e == self.epochs - 1]
+                for i in range(self.n)]
 
     def predict(self, X):
         return sum(m.predict(X) for m in self.models) / float(len(self.models))

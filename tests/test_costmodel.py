@@ -171,3 +171,24 @@ def test_hip_training_loop_equals_stepwise_training(gpu):
         grads, loss = b.backward_raw(acts, ys)
         b.adam(grads, 3e-3)
     assert torch.equal(a.params, b.params)
+
+
+@pytest.mark.gpu
+def test_ensemble_training_equals_member_training(gpu):
+    """pmx_mlp_train_ensemble (every kernel over all members at once) leaves
+    each member exactly where training it alone with pmx_mlp_train does."""
+    skip_if_prebuild()
+    import torch
+    from polymage_amd.costmodel import MLP, Ensemble, FEATURES
+    rng = np.random.default_rng(5)
+    X = rng.normal(0, 1, (300, len(FEATURES)))
+    y = X[:, 0] * X[:, 1] - np.abs(X[:, 2])
+    ens = Ensemble(n=3, epochs=30)
+    ens.fit(X, y)
+    for s in range(3):
+        alone = MLP(seed=s)
+        alone.fit(X, y, epochs=30, lr=3e-3)
+        assert torch.equal(alone.params, ens.models[s].params), s
+    p = ens.predict(X[:50])
+    q = sum(MLP.predict(m, X[:50]) for m in ens.models) / 3.0
+    assert np.array_equal(p, q)
