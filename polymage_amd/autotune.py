@@ -11,7 +11,8 @@ For every application:
    compilation), rank them with an ensemble of MLP cost models (HIP
    kernels) trained on the timing records of the *other* applications,
    compile and time the top k, refit with those k measurements, time the
-   next k, and so on (three rounds of 6 by default), keep the fastest
+   next k, and so on (two rounds of 10 by default: 10 compiles fit the
+   12 compile workers, so a round costs one compile latency), keep the fastest
    measured;
 3. report, per application: the exhaustive winner, the learned winner, the
    learned winner's time *in the exhaustive run* over the exhaustive best
@@ -56,7 +57,7 @@ def exhaustive(app, procs, db, runs=9, pool=None):
             'compile_s': t_compile, 'time_s': t_time, 'n': len(cands)}
 
 
-def learned(app, procs, train_records, k=6, rounds=3, runs=9, dup=20, pool=None):
+def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None):
     """The learned search: rank every candidate by the cost model trained on
     the other applications' timings, compile and time the top k, refit with
     those k measurements of this application (weighted by repetition), time
@@ -118,8 +119,8 @@ def main():
     ap.add_argument('--apps', default='unsharp,harris,bilateral,laplacian,campipe')
     ap.add_argument('--out', default='gpurun_out/tune')
     ap.add_argument('--procs', type=int, default=12)
-    ap.add_argument('--k', type=int, default=6)
-    ap.add_argument('--rounds', type=int, default=3)
+    ap.add_argument('--k', type=int, default=10)
+    ap.add_argument('--rounds', type=int, default=2)
     ap.add_argument('--runs', type=int, default=9)
     ap.add_argument('--reuse-db', default=None, help='timings.jsonl from an earlier run')
     ap.add_argument('--record', action='store_true', help='write polymage_amd/tuned.json')
