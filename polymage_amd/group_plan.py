@@ -726,6 +726,9 @@ class GroupPlan(object):
         shrinks with the reuse a block gives to the stage's stencil reach."""
         threads = int(self.opts.get('threads', 256))
         forced = self.opts.get('block')
+        # fixed cost of one sweep iteration, in the same units as a point's
+        # cost (8 + 2 per load)
+        it_cost = float(self.opts.get('iter_cost', 6))
         for s in self.stages:
             s.block = (1, 1)
             s.bdims = (s.nd - 2 if s.nd >= 2 else None, s.nd - 1 if s.nd >= 1 else None)
@@ -772,7 +775,7 @@ class GroupPlan(object):
                     items = outer * (-(-ext_r // U)) * (-(-ext_c // V))
                     iters = -(-items // threads)
                     taps = (U + 2.0 * rv) / U * (V + 2.0 * rh) / V
-                    cost = iters * (U * V * (8.0 + 2.0 * taps) + 6.0)
+                    cost = iters * (U * V * (8.0 + 2.0 * taps) + it_cost)
                     key = (cost, -U * V)
                     if best is None or key < best[0]:
                         best = (key, (U, V))

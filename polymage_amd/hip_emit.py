@@ -515,7 +515,11 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         pb = []
         for i in range(nd):
             st = _prod(cnt[i + 1:])
-            if i < nd - 1:
+            if cnt[i] == 1:
+                # a dimension of one point (a split level or channel): the
+                # index is 0, uniform, so what depends on it is scalar work
+                w3('const int j%d = 0;' % i)
+            elif i < nd - 1:
                 w3('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
                     i, rem, st, i, rem, i, st))
                 rem = 'k%d' % i

@@ -226,20 +226,68 @@ class Space(object):
 POW2_TILES = [(a, b) for a in (8, 16, 32, 64, 128, 256, 512) for b in (8, 16, 32, 64, 128, 256, 512)]
 
 
+_HALO = {}
+
+
+def row_halo(app):
+    """Points the widest materialised stage of the application's main
+    group adds to the tile along the innermost tiled dimension (unsharp's
+    vertical blur is computed on 256 + 4 columns of a 256-wide tile)."""
+    if app not in _HALO:
+        from .apps import workloads
+        from .hip_emit import plan_pipeline
+        from .group_plan import GroupPlan
+        from .analysis import PlanError
+        halo = 0
+        plans = []
+        for t in ((32, 256), (16, 64)):
+            try:
+                w = getattr(workloads, app)(tiles=t, with_data=False)
+                plans = [p for p in plan_pipeline(w.pipe)
+                         if isinstance(p, GroupPlan) and p.grid_dims]
+                break
+            except PlanError:
+                continue
+        if plans:
+            p = max(plans, key=lambda q: len(q.stages))
+            d, T = p.grid_dims[-1]
+            for st in p.stages:
+                if st.inline:
+                    continue
+                for i, al in enumerate(st.align):
+                    if al is not None and al[0] == d and al[1] == 1:
+                        halo = max(halo, st.static_ext[i] - T)
+        _HALO[app] = halo
+    return _HALO[app]
+
+
+def aligned_tiles(app):
+    """The power-of-two tiles with the innermost tile size reduced by the
+    application's halo, so the widest stage region is a power of two wide
+    and its sweep fills whole waves (unsharp: 16x252)."""
+    h = row_halo(app)
+    if h <= 0:
+        return []
+    return [(a, b - h) for (a, b) in POW2_TILES if b >= 32 and b - h > h]
+
+
 def default_space(app):
     """Tile sizes {8..512}^2 (powers of two, as the benchmark configuration
-    states) times the application's group sizes times workgroup sizes.
-    Schedules that do not fit (LDS) fail at planning and are skipped."""
+    states) and the same with the innermost size less the application's
+    halo (``aligned_tiles``), times the application's group sizes times
+    workgroup sizes.  Schedules that do not fit (LDS) fail at planning and
+    are skipped."""
+    tiles = POW2_TILES + aligned_tiles(app)
     if app == 'unsharp':
-        return Space(POW2_TILES, [1, 5], (128, 256, 512))
+        return Space(tiles, [1, 5], (128, 256, 512))
     if app == 'harris':
-        return Space(POW2_TILES, [1, 5, 11], (256, 512))
+        return Space(tiles, [1, 5, 11], (256, 512))
     if app == 'bilateral':
-        return Space(POW2_TILES, [1, 5], (128, 256))
+        return Space(tiles, [1, 5], (128, 256))
     if app == 'laplacian':
-        return Space(POW2_TILES, [1, 3, 5, 8], (256,))
+        return Space(tiles, [1, 3, 5, 8], (256,))
     if app == 'campipe':
-        return Space(POW2_TILES, [3, 5, 12, 40], (256,))
+        return Space(tiles, [3, 5, 12, 40], (256,))
     raise KeyError(app)
 
 
