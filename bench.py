@@ -201,8 +201,8 @@ def batch_bench(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py: this build measures one GPU only; --gpus %d is not supported '
-              '(multi-GPU sharding is described in DESIGN.md)' % args.gpus, file=sys.stderr)
+        print('bench.py measures one GPU; --gpus %d is not supported '
+              '(no sharding across GPUs)' % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe, args.steps)

@@ -117,7 +117,7 @@ def _default_key(app):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--apps', default='unsharp,harris,bilateral,laplacian,campipe')
-    ap.add_argument('--out', default='gpurun_out/tune')
+    ap.add_argument('--out', default='tune_out')
     ap.add_argument('--procs', type=int, default=12)
     ap.add_argument('--k', type=int, default=10)
     ap.add_argument('--rounds', type=int, default=2)
@@ -160,7 +160,7 @@ def main():
         ex_times = ex[app]['times']
         ex_best_key, ex_best = min(ex_times.items(), key=lambda kv: kv[1])
         le_best_key, le_best = min(le['times'].items(), key=lambda kv: kv[1])
-        # the model's pick judged on the exhaustive run's own measurements
+        # the model's pick as the exhaustive run itself measured it
         in_ex = [ex_times[kk] for kk in le['top'] if kk in ex_times]
         le_same = min(in_ex) if in_ex else None
         ex_total = ex[app]['compile_s'] + ex[app]['time_s']

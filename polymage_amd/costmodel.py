@@ -15,7 +15,7 @@ hand-written HIP kernels of libpmx_rt (``pmx_mlp_forward`` /
 ``pmx_mlp_backward`` / ``pmx_mlp_adam``; the training loop of a whole
 ensemble is one ``pmx_mlp_train_ensemble`` call).  It regresses log(ns per output
 pixel) from standardised features.  Candidates are ranked by predicted
-time; the exhaustive tuner remains the fallback and the judge of regret.
+time; the exhaustive tuner remains the fallback and the yardstick for regret.
 """
 
 import ctypes

@@ -25,7 +25,7 @@ def test_grouping_harris():
 
 
 def test_reference_trapezoid_harris():
-    # 11 stages, height 4, slopes +-1/3 -> every stage on 40x264
+    # 11 stages, height 4, slopes +-1/3 -> every stage on 40x264 (poly_schedule.py:429-536)
     pipe = _harris([32, 256])
     geo = reference_geometry(pipe, pipe.groups[0])
     assert geo['h'] == 4
