@@ -181,7 +181,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         e = None
         for d, q in enumerate(idx_strs):
             term = '(%s - %s)' % (q, R(p.region[d][0]))
-            e = term if e is None else '(%s * %d + %s)' % (e, plan.lds_ext(p)[d], term)
+            e = term if e is None else '(__mul24(%s, %d) + %s)' % (e, plan.lds_ext(p)[d], term)
         return e if e is not None else '0'
 
     # references that only run inside a Select branch: their global reads
@@ -272,6 +272,10 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                         terms.append('(%d * %s)' % (o, stride))
                 elif stride == 1:
                     terms.append('(%s)' % o)
+                elif isinstance(stride, int):
+                    # LDS strides and offsets are small: a 24-bit multiply
+                    # (full rate; a 32-bit one issues at a quarter)
+                    terms.append('__mul24(%s, %d)' % (o, stride))
                 else:
                     terms.append('((%s) * %s)' % (o, stride))
             if const or not terms:
@@ -299,7 +303,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                     terms = []
                     for j, (k, off) in enumerate(sh):
                         pbk = self.pb[k] if k != 'const' else '0'
-                        terms.append('(%s - %s) * %d' % (pbk, R(p.region[j][0]), W[j]))
+                        terms.append('__mul24(%s - %s, %d)' % (pbk, R(p.region[j][0]), W[j]))
                     return 'const int %s = %s;' % (nm, ' + '.join(terms) if terms else '0')
                 nm = self._base(('l', p.name, kinds), mk)
                 for j, (k, off) in enumerate(sh):
@@ -424,7 +428,8 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             W = [_prod(ext[j + 1:]) for j in range(s.nd)]
 
             def mk(nm):
-                terms = ['(%s - %s) * %d' % (item.pb[j], R(s.region[j][0]), W[j]) for j in range(s.nd)]
+                terms = ['__mul24(%s - %s, %d)' % (item.pb[j], R(s.region[j][0]), W[j])
+                         for j in range(s.nd)]
                 return 'const int %s = %s;' % (nm, ' + '.join(terms) if terms else '0')
             nm = item._base(('ls', s.name), mk)
             off = item._off([(item.d[j], W[j]) for j in range(s.nd)], None)
@@ -520,8 +525,15 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 # index is 0, uniform, so what depends on it is scalar work
                 w3('const int j%d = 0;' % i)
             elif i < nd - 1:
-                w3('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
-                    i, rem, st, i, rem, i, st))
+                mg = _magic24(cnt[i] * st, st)
+                if mg:
+                    # it / st as a 24-bit multiply and a shift, exact for
+                    # every dividend this sweep produces
+                    w3('const int j%d = (int)(__umul24((unsigned)%s, %du) >> %d); '
+                       'const int k%d = %s - __mul24(j%d, %d);' % (i, rem, mg[0], mg[1], i, rem, i, st))
+                else:
+                    w3('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (
+                        i, rem, st, i, rem, i, st))
                 rem = 'k%d' % i
             else:
                 w3('const int j%d = %s;' % (i, rem))
@@ -647,6 +659,23 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     return wrap_kernel(kname, params, buffers, writes, threads, plan.lds_bytes, L, roll=roll), info
 
 
+
+
+def _magic24(n, d):
+    """(m, s) with (x * m) >> s == x // d for every 0 <= x < n, where x * m
+    stays below 2**32 and m below 2**24 (so a 24-bit multiply computes it
+    exactly), or None."""
+    if d <= 1 or n <= 0 or n >= (1 << 24):
+        return None
+    import numpy as np
+    x = np.arange(n, dtype=np.uint64)
+    for sh in range(1, 33):
+        m = -(-(1 << sh) // d)
+        if m >= (1 << 24) or (n - 1) * m >= (1 << 32):
+            break
+        if np.array_equal((x * np.uint64(m)) >> np.uint64(sh), x // np.uint64(d)):
+            return m, sh
+    return None
 
 
 def kernel_args(params, buffers, writes):
