@@ -123,3 +123,26 @@ def test_recomputed_stage_keeps_its_float_rounding():
     kernel_emu.build(pipe).run_host({'R': 61, 'img': v, 'b': g})
     cemit.build([b], 'ri_oracle')({'R': 61, 'img': v, 'b': o})
     assert bits_equal(g, o)
+
+
+def test_size_limits_and_bad_batches_are_errors():
+    """Sizes whose arrays overflow 32-bit indexing are refused before any
+    memory is touched (status -75 and a message naming the array), and an
+    empty batch is refused (-22): the C ABI reports instead of faulting."""
+    import ctypes
+    pd = {}
+    out = unsharp_mask_full(pd)
+    pipe = build(out, pd, {'R': 64, 'C': 64}, 'unsharp', [16, 32], group_size=5)
+    ep = kernel_emu.build(pipe)
+    dev = ep.lib['pipeline_unsharp_dev']
+    dev.restype = ctypes.c_int
+    big = 40000
+    rc = dev(ctypes.c_int(big), ctypes.c_int(big), ctypes.c_float(0.001), ctypes.c_float(3.0),
+             ctypes.c_void_p(0), ctypes.c_void_p(0), ctypes.c_void_p(0))
+    assert rc == -75
+    assert b'too large for 32-bit indexing' in ep.lib.pmx_last_error()
+    create = ep.lib['pipeline_unsharp_batch_create']
+    create.restype = ctypes.c_int
+    h = ctypes.c_void_p()
+    rc = create(ctypes.c_int(0), None, None, None, None, None, None, ctypes.byref(h))
+    assert rc == -22 and b'no frames' in ep.lib.pmx_last_error()
