@@ -225,7 +225,8 @@ class GroupPlan(object):
 
         A stage that is not a live-out, has one unconditional definition of
         at most CHEAP_OPS operations and reads its producers only at its own
-        point (``Ixx = Ix * Ix``) is substituted into its in-group consumers
+        point (``Ixx = Ix * Ix``; a plane copy ``r_r(x, y) =
+        deinterleaved(1, x, y)`` too) is substituted into its in-group consumers
         when one of them reads it with an offset (``Sxx`` sums Ixx over a
         3x3 window): the consumer then reads the producers at the shifted
         points.  The value is unchanged (the substituted expression is cast
@@ -261,7 +262,7 @@ class GroupPlan(object):
                 if nops > CHEAP_OPS:
                     continue
                 refs = e.collect(Reference)
-                if not refs or any(list(r.arguments) != list(p.func.variables) for r in refs):
+                if not refs or not all(self._own_point(r, p) for r in refs):
                     continue
                 # its consumers in the group; at least one reads with an offset
                 cons = []
@@ -302,6 +303,21 @@ class GroupPlan(object):
                 changed = True
                 break
         self.by_comp = dict((s.comp, s) for s in self.stages)
+
+    @staticmethod
+    def _own_point(r, p):
+        """True when reference r reads its function at p's own point: its
+        arguments are p's variables in order, with integer constants
+        allowed in between (a plane of a multi-channel producer, like a
+        camera pipeline's ``r_r(x, y) = deinterleaved(1, x, y)``)."""
+        from .expr_ast import Value
+        vs = []
+        for a in r.arguments:
+            if isinstance(a, Value) and isinstance(a.value, int):
+                continue
+            vs.append(a)
+        return len(vs) == len(p.func.variables) and \
+            all(a is v for a, v in zip(vs, p.func.variables))
 
     def _taps_inside(self, p, part):
         """True when every point at which an in-group consumer reads p
