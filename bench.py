@@ -86,31 +86,54 @@ def device_copy_gbs(torch, dev, mib=512, reps=10):
     return round(2 * n * 4 / (ms / 1e3) / 1e9, 1)
 
 
-def cpu_baseline(w, budget_s):
-    """Oracle port (C++/OpenMP, reference compile flags) on host cores."""
-    import numpy as np
-    from oracle import cemit
-    threads = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault('OMP_NUM_THREADS', str(threads))
-    os.environ.setdefault('OMP_PROC_BIND', 'close')
-    prog = cemit.build([w.out_func], 'bench_cpu_%s' % w.name, flags=cemit.REFERENCE_FLAGS)
-    vals = dict(w.values)
-    vals[w.out_func.name] = np.zeros(w.out_shape, w.out_func.typ.np_dtype())
-    prog(vals)     # warm-up
+def _time_cpu(fn, budget_s):
+    """Min wall time of up to 200 calls of fn within budget_s seconds."""
+    fn()     # warm-up
     best = None
     runs = 0
     t_start = time.perf_counter()
     while runs < 200 and time.perf_counter() - t_start < budget_s:
         t0 = time.perf_counter()
-        prog(vals)
+        fn()
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
         runs += 1
+    return best, runs, time.perf_counter() - t_start
+
+
+def cpu_baseline(w, budget_s):
+    """The CPU path on the host cores: for Harris at its benchmark size,
+    the reference's own generated refHarris (oracle/_ref, built with the
+    reference's compile flags); otherwise the oracle's C++/OpenMP port of
+    the same pipeline (reference compile flags)."""
+    import numpy as np
+    threads = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault('OMP_NUM_THREADS', str(threads))
+    os.environ.setdefault('OMP_PROC_BIND', 'close')
+    ref_so = os.path.join(ROOT, 'oracle', '_ref', 'libref_harris_bench.so')
+    if w.name == 'harris' and os.path.exists(ref_so):
+        import ctypes
+        lib = ctypes.CDLL(ref_so)
+        R, C = w.values['R'], w.values['C']
+        img = w.values['img']
+        out = np.zeros((R + 2, C + 2), np.float32)
+        args = (ctypes.c_int(C), ctypes.c_int(R), ctypes.c_void_p(img.ctypes.data),
+                ctypes.c_void_p(out.ctypes.data))
+        best, runs, spent = _time_cpu(lambda: lib.ref_harris(*args), budget_s)
+        return {'value': round(w.mpix / best, 3), 'unit': 'MPix/s',
+                'cores': int(os.environ['OMP_NUM_THREADS']), 'kind': 'reference',
+                'sample': '%d full-size runs (%.1f s) of the reference\'s generated refHarris '
+                          '(ipp_bench/harris/harris_function.cpp, 32x256 tiles, float literals; '
+                          'g++ -O3 -fopenmp -march=x86-64-v3 -ftree-vectorize), min time' % (runs, spent)}
+    from oracle import cemit
+    prog = cemit.build([w.out_func], 'bench_cpu_%s' % w.name, flags=cemit.REFERENCE_FLAGS)
+    vals = dict(w.values)
+    vals[w.out_func.name] = np.zeros(w.out_shape, w.out_func.typ.np_dtype())
+    best, runs, spent = _time_cpu(lambda: prog(vals), budget_s)
     return {'value': round(w.mpix / best, 3), 'unit': 'MPix/s', 'cores': int(os.environ['OMP_NUM_THREADS']),
             'kind': 'port',
-            'sample': '%d full-size runs (%.1f s of CPU time) of the oracle C++/OpenMP port of the '
-                      'same pipeline (g++ -O3 -fopenmp -march=native), min time' % (
-                          runs, time.perf_counter() - t_start)}
+            'sample': '%d full-size runs (%.1f s) of the oracle C++/OpenMP port of the '
+                      'same pipeline (g++ -O3 -fopenmp -march=native), min time' % (runs, spent)}
 
 
 def schedule_for(args):
