@@ -41,6 +41,9 @@ def parse():
                     help='extra code-generation option (e.g. no_stage=1, block=8,1)')
     ap.add_argument('--schedule', default='learned', choices=['learned', 'exhaustive', 'default'],
                     help='schedule recorded by the autotuner for this workload')
+    ap.add_argument('--literal-mode', default='double', choices=['double', 'float12'],
+                    help="float literals as C doubles (the reference's code generator) or as "
+                         "12-digit float literals (its checked-in generated files)")
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--traffic', default=os.path.join(ROOT, 'profiles', 'traffic.json'),
@@ -126,7 +129,9 @@ def cpu_baseline(w, budget_s):
                           '(ipp_bench/harris/harris_function.cpp, 32x256 tiles, float literals; '
                           'g++ -O3 -fopenmp -march=x86-64-v3 -ftree-vectorize), min time' % (runs, spent)}
     from oracle import cemit
-    prog = cemit.build([w.out_func], 'bench_cpu_%s' % w.name, flags=cemit.REFERENCE_FLAGS)
+    mode = w.pipe.literal_mode
+    prog = cemit.build([w.out_func], 'bench_cpu_%s_%s' % (w.name, mode), flags=cemit.REFERENCE_FLAGS,
+                       literal_mode=mode)
     vals = dict(w.values)
     vals[w.out_func.name] = np.zeros(w.out_shape, w.out_func.typ.np_dtype())
     best, runs, spent = _time_cpu(lambda: prog(vals), budget_s)
@@ -168,6 +173,7 @@ def batch_bench(args):
     tiles = sched['tiles']
     gopts = {'threads': sched['threads']} if sched.get('threads') else None
     kw = {'group_size': sched['group_size']} if sched.get('group_size') else {}
+    kw['literal_mode'] = args.literal_mode
     w = workloads.campipe(tiles=tiles, gpu_options=gopts, with_data=False, **kw)
     bp = runtime.build(w.pipe)
     dev = torch.device('cuda:0')
@@ -265,6 +271,7 @@ def main():
     if sched.get('threads'):
         gopts['threads'] = sched['threads']
     kw = {'group_size': sched['group_size']} if sched.get('group_size') else {}
+    kw['literal_mode'] = args.literal_mode
     w = getattr(workloads, args.workload)(tiles=tiles, gpu_options=gopts or None, **kw)
     bp = runtime.build(w.pipe)
 
@@ -339,6 +346,7 @@ def main():
         'data': 'synthetic (seeded smooth random field, rotated over %d HBM-resident buffer sets)' % nsets,
         'config': {'workload': '%s %s' % (w.name, 'x'.join(str(s) for s in w.out_shape)),
                    'tiles': list(tiles), 'kernels': nk, 'schedule': sched_info,
+                   'literal_mode': args.literal_mode,
                    'gpu_options': dict((k, list(v) if isinstance(v, tuple) else v)
                                        for k, v in gopts.items()),
                    'wall_s': round(wall, 4)},

@@ -36,13 +36,13 @@ def _pipe(out, pd, sizes, name, tiles, group_size, literal_mode='double', gpu_op
 
 
 def harris(rows=2048, cols=2048, tiles=(32, 256), gpu_options=None, with_data=True,
-           group_size=11):
+           group_size=11, literal_mode='double'):
     from .harris import harris_pipe
     pd = {}
     out = harris_pipe(pd)
     R, C = rows - 2, cols - 2
     pipe = _pipe(out, pd, {'R': R, 'C': C}, 'harris', list(tiles), group_size,
-                 gpu_options=gpu_options)
+                 gpu_options=gpu_options, literal_mode=literal_mode)
     vals = {'R': R, 'C': C}
     if with_data:
         vals['img'] = inputs.harris_input(rows, cols)
@@ -53,13 +53,13 @@ def harris(rows=2048, cols=2048, tiles=(32, 256), gpu_options=None, with_data=Tr
 
 
 def unsharp(rows=2848, cols=4256, tiles=(32, 256), gpu_options=None, with_data=True,
-            group_size=5):
+            group_size=5, literal_mode='double'):
     from .unsharp import unsharp_mask_full
     pd = {}
     out = unsharp_mask_full(pd)
     R, C = rows - 4, cols - 4
     pipe = _pipe(out, pd, {'R': R, 'C': C}, 'unsharp', list(tiles), group_size,
-                 gpu_options=gpu_options)
+                 gpu_options=gpu_options, literal_mode=literal_mode)
     vals = {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0}
     if with_data:
         vals['img'] = inputs.unsharp_input(rows, cols)
@@ -69,12 +69,12 @@ def unsharp(rows=2848, cols=4256, tiles=(32, 256), gpu_options=None, with_data=T
 
 
 def bilateral(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
-              group_size=5):
+              group_size=5, literal_mode='double'):
     from .bilateral import bilateral_grid_full
     pd = {}
     out = bilateral_grid_full(pd)
     pipe = _pipe(out, pd, {'rows': rows, 'cols': cols}, 'bilateral', list(tiles), group_size,
-                 gpu_options=gpu_options)
+                 gpu_options=gpu_options, literal_mode=literal_mode)
     vals = {'rows': rows, 'cols': cols}
     if with_data:
         vals['input'] = inputs.bilateral_input(rows, cols)
@@ -83,7 +83,7 @@ def bilateral(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
 
 
 def laplacian(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
-              group_size=5, K=8, J=4):
+              group_size=5, K=8, J=4, literal_mode='double'):
     """Local Laplacian, uint16 RGB, K intensity levels, J pyramid levels,
     alpha = 1/(K-1), beta = 1; the input is padded by edge replication
     inside the pipeline so the output has the input's size."""
@@ -93,7 +93,7 @@ def laplacian(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
     lo, hi = pd['pad']
     R, C = rows + lo + hi, cols + lo + hi
     pipe = _pipe(out, pd, {'R': R, 'C': C}, 'laplacian', list(tiles), group_size,
-                 gpu_options=gpu_options)
+                 gpu_options=gpu_options, literal_mode=literal_mode)
     vals = {'R': R, 'C': C, 'alpha': 1.0 / (K - 1), 'beta': 1.0}
     if with_data:
         vals['img_colour'] = inputs.laplacian_input(rows, cols)
@@ -103,14 +103,14 @@ def laplacian(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
 
 
 def campipe(rows=1216, cols=1664, tiles=(16, 64), gpu_options=None, with_data=True,
-            group_size=40):
+            group_size=40, literal_mode='double'):
     """One frame of the batched camera pipeline (uint16 10-bit GRBG Bayer,
     3700 K, gamma 2.2, contrast 50, 3x3 sharpen, uint8 RGB out)."""
     from .campipe import camera_pipe, MATRIX_3200, MATRIX_7000
     pd = {}
     out = camera_pipe(pd, variant='batch')
     pipe = _pipe(out, pd, {'R': rows, 'C': cols}, 'campipe', list(tiles), group_size,
-                 gpu_options=gpu_options)
+                 gpu_options=gpu_options, literal_mode=literal_mode)
     vals = {'R': rows, 'C': cols, 'colour_temp': 3700.0, 'gamma': 2.2, 'contrast': 50.0,
             'matrix_3200': np.array(MATRIX_3200, np.float32),
             'matrix_7000': np.array(MATRIX_7000, np.float32)}
