@@ -101,3 +101,28 @@ def test_offset_inlining_needs_taps_inside_the_case():
     # b goes, then a (now read with offsets by c, unconditionally defined)
     assert getattr(plan, 'offset_inlined', []) == ['b', 'a']
     assert [s.name for s in plan.stages] == ['c']
+
+
+def test_magic24_division_is_exact_over_the_sweep_range():
+    """The sweep's item decomposition replaces x / d by a 24-bit multiply
+    and a shift only when that is exact for every dividend the sweep
+    produces; brute force over the whole range, including its last value."""
+    import numpy as np
+    from polymage_amd.hip_emit import _magic24
+    for n, d in [(1122, 66), (520, 260), (4096, 162), (65536, 1188), (33 * 19, 19), (7, 3)]:
+        m, s = _magic24(n, d)
+        x = np.arange(n, dtype=np.uint64)
+        assert m < (1 << 24) and (n - 1) * m < (1 << 32)
+        assert np.array_equal((x * np.uint64(m)) >> np.uint64(s), x // np.uint64(d))
+    assert _magic24(100000, 7) is None          # no 24-bit multiplier reaches that range
+    assert _magic24(10, 1) is None
+
+
+def test_halo_aligned_tiles():
+    """unsharp's vertical blur adds 2 columns on each side of a tile, Harris'
+    Sobel 1: the aligned tiles make the widest stage a power of two wide."""
+    from polymage_amd import tuner
+    assert tuner.row_halo('unsharp') == 4 and tuner.row_halo('harris') == 2
+    assert (16, 252) in tuner.aligned_tiles('unsharp')
+    assert (16, 510) in tuner.aligned_tiles('harris')
+    assert tuner.aligned_tiles('bilateral') == []
