@@ -34,7 +34,8 @@ def parse():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--workload', default='unsharp',
-                    choices=['unsharp', 'harris', 'bilateral', 'laplacian', 'campipe48'])
+                    choices=['unsharp', 'harris', 'bilateral', 'laplacian', 'campipe48',
+                             'interpolate'])
     ap.add_argument('--tiles', default=None, help='e.g. 32x256')
     ap.add_argument('--threads', type=int, default=None)
     ap.add_argument('--gpu-opt', action='append', default=[], metavar='KEY=VALUE',

@@ -122,8 +122,30 @@ def campipe(rows=1216, cols=1664, tiles=(16, 64), gpu_options=None, with_data=Tr
                     dtype='u8')
 
 
+def interpolate(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
+                group_size=5, literal_mode='double'):
+    """Multiscale interpolation (10 pyramid levels) of an RGBA float image
+    with a one-pixel ghost border; RGB float out.  Colour and alpha are
+    smooth random fields, alpha kept in [0.05, 1] so the normalisation
+    never divides by zero."""
+    from .interpolate import interpolate as interp
+    pd = {}
+    out = interp(pd)
+    R, C = rows, cols
+    pipe = _pipe(out, pd, {'R': R, 'C': C}, 'interpolate', list(tiles), group_size,
+                 gpu_options=gpu_options, literal_mode=literal_mode)
+    vals = {'R': R, 'C': C}
+    if with_data:
+        g = inputs.rng(6)
+        img = inputs.smooth_field(g, R + 2, C + 2, 4)
+        img[3] = 0.05 + 0.95 * img[3]
+        vals['img'] = np.ascontiguousarray(img, np.float32)
+    ideal = 4 * (R + 2) * (C + 2) * 4 + 3 * R * C * 4
+    return Workload('interpolate', pipe, out, vals, (3, R, C), R * C / 1e6, ideal)
+
+
 DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64),
-                 'laplacian': (32, 64), 'campipe': (16, 64)}
+                 'laplacian': (32, 64), 'campipe': (16, 64), 'interpolate': (32, 64)}
 
 
 def prebuild_default():
@@ -131,7 +153,7 @@ def prebuild_default():
     autotuner recorded, so benchmark runs find them in the kernel cache."""
     from .. import runtime, tuner
     fns = {'harris': harris, 'unsharp': unsharp, 'bilateral': bilateral,
-           'laplacian': laplacian, 'campipe': campipe}
+           'laplacian': laplacian, 'campipe': campipe, 'interpolate': interpolate}
     for name, fn in fns.items():
         w = fn(with_data=False)
         runtime.build(w.pipe)

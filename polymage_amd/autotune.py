@@ -110,7 +110,7 @@ def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None
 def _default_key(app):
     from .apps.workloads import DEFAULT_TILES
     from .tuner import Candidate
-    gs = {'harris': 11, 'unsharp': 5, 'bilateral': 5, 'laplacian': 5, 'campipe': 40}[app]
+    gs = {'harris': 11, 'campipe': 40}.get(app, 5)
     return Candidate(DEFAULT_TILES.get(app, (32, 64)), gs, 256).key
 
 

@@ -288,6 +288,8 @@ def default_space(app):
         return Space(tiles, [1, 3, 5, 8], (256,))
     if app == 'campipe':
         return Space(tiles, [3, 5, 12, 40], (256,))
+    if app == 'interpolate':
+        return Space(tiles, [1, 5, 8], (256,))
     raise KeyError(app)
 
 

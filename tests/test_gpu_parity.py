@@ -179,6 +179,35 @@ def _campipe_vals(img):
             'matrix_7000': np.array(MATRIX_7000, np.float32)}
 
 
+def _interpolate(R, C, tiles, gs=5, seed=11):
+    from polymage_amd.apps.interpolate import interpolate
+    pd = {}
+    out = interpolate(pd)
+    pipe = build(out, pd, {'R': R, 'C': C}, 'interpolate', tiles, group_size=gs)
+    img = inputs.smooth_field(np.random.default_rng(seed), R + 2, C + 2, 4)
+    img[3] = 0.05 + 0.95 * img[3]
+    img = np.ascontiguousarray(img, np.float32)
+    g = np.full((3, R, C), -7.0, np.float32)
+    o = g.copy()
+    gpu_build(pipe).run_host({'R': R, 'C': C, 'img': img, 'interpolate': g})
+    cemit.build([out], 'interp_oracle')({'R': R, 'C': C, 'img': img, 'interpolate': o})
+    return g, o
+
+
+@pytest.mark.parametrize("R,C,tiles,gs", [(70, 90, [16, 32], 5), (301, 517, [16, 64], 5),
+                                          (130, 94, [32, 64], 8)])
+def test_interpolate_bitexact(gpu, R, C, tiles, gs):
+    print("case interpolate", R, C, tiles, gs, flush=True)
+    g, o = _interpolate(R, C, tiles, gs)
+    assert bits_equal(g, o)
+
+
+def test_interpolate_full_size(gpu):
+    print("case interpolate 2560x1536", flush=True)
+    g, o = _interpolate(2560, 1536, [32, 64], seed=1006)
+    assert bits_equal(g, o)
+
+
 @pytest.mark.parametrize("variant,gs", [('reference', 5), ('batch', 5), ('batch', 40)])
 def test_campipe_bitexact(gpu, variant, gs):
     from polymage_amd.apps.campipe import camera_pipe
