@@ -35,7 +35,7 @@ def parse():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--workload', default='unsharp',
                     choices=['unsharp', 'harris', 'bilateral', 'laplacian', 'campipe48',
-                             'interpolate'])
+                             'interpolate', 'blend'])
     ap.add_argument('--tiles', default=None, help='e.g. 32x256')
     ap.add_argument('--threads', type=int, default=None)
     ap.add_argument('--gpu-opt', action='append', default=[], metavar='KEY=VALUE',

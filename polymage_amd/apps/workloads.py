@@ -144,8 +144,31 @@ def interpolate(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_dat
     return Workload('interpolate', pipe, out, vals, (3, R, C), R * C / 1e6, ideal)
 
 
+def blend(rows=2560, cols=1536, tiles=(16, 64), gpu_options=None, with_data=True,
+          group_size=5, literal_mode='double'):
+    """Laplacian-pyramid blending (4 levels) of two float RGB images under
+    a float mask; float RGB out on the window the reference defines (a
+    31-point frame is not written)."""
+    from .pyramid_blend import pyramid_blend
+    pd = {}
+    out = pyramid_blend(pd)
+    R, C = rows, cols
+    pipe = _pipe(out, pd, {'R': R, 'C': C}, 'blend', list(tiles), group_size,
+                 gpu_options=gpu_options, literal_mode=literal_mode)
+    vals = {'R': R, 'C': C}
+    if with_data:
+        g = inputs.rng(7)
+        vals['img1'] = np.ascontiguousarray(inputs.smooth_field(g, R, C, 3), np.float32)
+        vals['img2'] = np.ascontiguousarray(inputs.smooth_field(g, R, C, 3), np.float32)
+        vals['mask'] = np.ascontiguousarray(inputs.smooth_field(g, R, C, 1)[0], np.float32)
+    wr, wc = R - 60, C - 60           # rows/cols 31 .. R-30 are written
+    ideal = (2 * 3 + 1) * R * C * 4 + 3 * wr * wc * 4
+    return Workload('blend', pipe, out, vals, (3, R, C), wr * wc / 1e6, ideal)
+
+
 DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64),
-                 'laplacian': (32, 64), 'campipe': (16, 64), 'interpolate': (32, 64)}
+                 'laplacian': (32, 64), 'campipe': (16, 64), 'interpolate': (32, 64),
+                 'blend': (16, 64)}
 
 
 def prebuild_default():
@@ -153,7 +176,8 @@ def prebuild_default():
     autotuner recorded, so benchmark runs find them in the kernel cache."""
     from .. import runtime, tuner
     fns = {'harris': harris, 'unsharp': unsharp, 'bilateral': bilateral,
-           'laplacian': laplacian, 'campipe': campipe, 'interpolate': interpolate}
+           'laplacian': laplacian, 'campipe': campipe, 'interpolate': interpolate,
+           'blend': blend}
     for name, fn in fns.items():
         w = fn(with_data=False)
         runtime.build(w.pipe)

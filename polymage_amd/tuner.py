@@ -290,6 +290,8 @@ def default_space(app):
         return Space(tiles, [3, 5, 12, 40], (256,))
     if app == 'interpolate':
         return Space(tiles, [1, 5, 8], (256,))
+    if app == 'blend':
+        return Space(tiles, [1, 3, 5, 6], (256,))
     raise KeyError(app)
 
 

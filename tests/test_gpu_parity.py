@@ -208,6 +208,37 @@ def test_interpolate_full_size(gpu):
     assert bits_equal(g, o)
 
 
+def _blend(R, C, tiles, gs=5, seed=12):
+    from polymage_amd.apps.pyramid_blend import pyramid_blend
+    pd = {}
+    out = pyramid_blend(pd)
+    pipe = build(out, pd, {'R': R, 'C': C}, 'blend', tiles, group_size=gs)
+    g_ = np.random.default_rng(seed)
+    vals = {'R': R, 'C': C,
+            'img1': np.ascontiguousarray(inputs.smooth_field(g_, R, C, 3), np.float32),
+            'img2': np.ascontiguousarray(inputs.smooth_field(g_, R, C, 3), np.float32),
+            'mask': np.ascontiguousarray(inputs.smooth_field(g_, R, C, 1)[0], np.float32)}
+    g = np.full((3, R, C), -7.0, np.float32)
+    o = g.copy()
+    gpu_build(pipe).run_host(dict(vals, blend=g))
+    cemit.build([out], 'blend_oracle')(dict(vals, blend=o))
+    return g, o
+
+
+@pytest.mark.parametrize("R,C,tiles,gs", [(96, 130, [16, 32], 5), (257, 391, [16, 64], 3),
+                                          (150, 203, [16, 32], 6)])
+def test_blend_bitexact(gpu, R, C, tiles, gs):
+    print("case blend", R, C, tiles, gs, flush=True)
+    g, o = _blend(R, C, tiles, gs)
+    assert bits_equal(g, o)        # the written window and the untouched frame
+
+
+def test_blend_full_size(gpu):
+    print("case blend 2560x1536", flush=True)
+    g, o = _blend(2560, 1536, [16, 64], seed=1007)
+    assert bits_equal(g, o)
+
+
 @pytest.mark.parametrize("variant,gs", [('reference', 5), ('batch', 5), ('batch', 40)])
 def test_campipe_bitexact(gpu, variant, gs):
     from polymage_amd.apps.campipe import camera_pipe
