@@ -146,3 +146,30 @@ def test_size_limits_and_bad_batches_are_errors():
     h = ctypes.c_void_p()
     rc = create(ctypes.c_int(0), None, None, None, None, None, None, ctypes.byref(h))
     assert rc == -22 and b'no frames' in ep.lib.pmx_last_error()
+
+
+@pytest.mark.parametrize("app,R,C,tiles,gs", [('interpolate', 70, 90, [16, 32], 5),
+                                              ('blend', 96, 130, [16, 32], 5)])
+def test_emulated_pyramid_apps(app, R, C, tiles, gs):
+    """The reference's interpolation and pyramid-blending apps: many small
+    groups over scaled accesses and Selects on parity, bit for bit."""
+    g_ = np.random.default_rng(R + C)
+    pd = {}
+    if app == 'interpolate':
+        from polymage_amd.apps.interpolate import interpolate
+        out = interpolate(pd)
+        vals = {'R': R, 'C': C, 'img': g_.random((4, R + 2, C + 2)).astype(np.float32)}
+        oname = 'interpolate'
+    else:
+        from polymage_amd.apps.pyramid_blend import pyramid_blend
+        out = pyramid_blend(pd)
+        vals = {'R': R, 'C': C, 'img1': g_.random((3, R, C)).astype(np.float32),
+                'img2': g_.random((3, R, C)).astype(np.float32),
+                'mask': g_.random((R, C)).astype(np.float32)}
+        oname = 'blend'
+    pipe = build(out, pd, {'R': R, 'C': C}, app, tiles, group_size=gs)
+    g = np.full((3, R, C), -7.0, np.float32)
+    o = g.copy()
+    kernel_emu.build(pipe).run_host(dict(vals, **{oname: g}))
+    cemit.build([out], app + '_oracle')(dict(vals, **{oname: o}))
+    assert bits_equal(g, o)
