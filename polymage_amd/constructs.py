@@ -307,9 +307,23 @@ class Reduction(Function):
     @defn.setter
     def defn(self, _def):
         assert self._body == []
+        bound = set(self._redVariables)
         for case in _def:
             case = Value.numericToValue(case)
             assert isinstance(case, (Case, Reduce)), str(case)
+            # the accumulation runs over the reduction domain only, so in
+            # the Reduce (target and value) a variable that is not a
+            # reduction variable has no value (max_filter's vert_log,
+            # polymage_maxfilter.py:60-66, reduces into vert_log(x, y, c, t)
+            # over (x, rx, c, ry)); a Case condition may name the function's
+            # own variables, which select target points (the bilateral grid's
+            # c == 0 / c == 1)
+            red = case.expression if isinstance(case, Case) else case
+            free = [v for v in red.collect(Variable) if v not in bound]
+            if free:
+                raise TypeError("Reduction %s: %s not among its reduction variables %s"
+                                % (self.name, ', '.join(sorted(v.name for v in free)),
+                                   [v.name for v in self._redVariables]))
             self._body.append(case)
 
     def hasBoundedIntegerDomain(self):

@@ -134,3 +134,27 @@ def test_stencil_expansion():
         '((((f1(x, (y + -1)) * -1) + (f1(x, y) * 2)) + (f1(x, (y + 1)) * -3.8)) + (f1((x + 1), y) * -1))'
     # a Stencil inside a Case is expanded when the Case is built
     assert str(f1.defn[0].expression).startswith('((input((x + -1)')
+
+
+def test_reduction_rejects_unbound_variables():
+    """A Reduce may only name its reduction variables: the accumulation
+    loops over the reduction domain, so any other variable has no value.
+    The reference's max_filter writes such a reduction
+    (polymage_maxfilter.py:60-66: vert_log(x, y, c, t) over (x, rx, c, ry))."""
+    import pytest
+    R = Parameter(Int, "R")
+    x, y, rx = Variable(Int, "x"), Variable(Int, "y"), Variable(Int, "rx")
+    img = Image(Float, "img", [R, R])
+    rows = Interval(Int, 0, R - 1)
+    ok = Reduction(([x], [rows]), ([x, rx], [rows, rows]), Float, "rowmax")
+    ok.defn = [Reduce(ok(x), img(x, rx), Op.Max)]
+    bad = Reduction(([x, y], [rows, rows]), ([x, rx], [rows, rows]), Float, "bad")
+    with pytest.raises(TypeError, match="y not among its reduction variables"):
+        bad.defn = [Reduce(bad(x, y), img(x, rx), Op.Max)]
+    bad2 = Reduction(([x], [rows]), ([x, rx], [rows, rows]), Float, "bad2")
+    with pytest.raises(TypeError, match="y not among"):
+        bad2.defn = [Case(Condition(x, '>', 0), Reduce(bad2(x), img(y, rx), Op.Sum))]
+    # conditions on the function's own variables select target points
+    sel = Reduction(([y, x], [Interval(Int, 0, 1), rows]), ([x, rx], [rows, rows]), Float, "sel")
+    sel.defn = [Case(Condition(y, '==', 0), Reduce(sel(0, x), img(x, rx), Op.Sum)),
+                Case(Condition(y, '==', 1), Reduce(sel(1, x), 1, Op.Sum))]
