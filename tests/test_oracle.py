@@ -158,6 +158,46 @@ def test_oracle_vs_eval64_float_pipelines():
                   'bilateral_oracle_full', (1, 128, 96))
 
 
+def test_oracle_vs_eval64_pyramid_pipelines():
+    """Multiscale interpolation and pyramid blending (float32) against the
+    float64 evaluator; the blend output is stored over its window
+    (store index = point - lower bound, codegen.py:320-347)."""
+    from polymage_amd.apps import inputs
+    from polymage_amd.apps.interpolate import interpolate
+    from polymage_amd.apps.pyramid_blend import pyramid_blend
+    R, C = 128, 96
+    img = inputs.smooth_field(np.random.default_rng(5), R + 2, C + 2, 4)
+    _eval64_check(interpolate({}, levels=5), {'R': R, 'C': C, 'img': img}, 'interpolate',
+                  'interp_oracle_eval64', (3, R, C))
+    a = inputs.smooth_field(np.random.default_rng(6), R, C, 3)
+    b = inputs.smooth_field(np.random.default_rng(7), R, C, 3)
+    m = inputs.smooth_field(np.random.default_rng(8), R, C, 1)[0]
+    # level-0 window of 4 levels: [31, R - 30] x [31, C - 30]
+    _eval64_check(pyramid_blend({}), {'R': R, 'C': C, 'img1': a, 'img2': b, 'mask': m}, 'blend',
+                  'blend_oracle_eval64', (3, R - 60, C - 60))
+
+
+def test_oracle_vs_eval64_local_laplacian():
+    """Local Laplacian (uint16): the float64 evaluator agrees with the
+    oracle except where the float32 intensity-level index of a pixel
+    (a data-dependent LUT and level choice) falls on the other side of an
+    integer than in float64."""
+    from oracle import eval64
+    from polymage_amd.apps import inputs
+    from polymage_amd.apps.local_laplacian import local_laplacian
+    pd = {}
+    out = local_laplacian(pd, K=8, J=4, io='u16', lut_hi_inclusive=True)
+    lo, hi = pd['pad']
+    rows, cols = 128, 96
+    vals = {'R': rows + lo + hi, 'C': cols + lo + hi, 'alpha': 1 / 7., 'beta': 1.0,
+            'img_colour': inputs.laplacian_input(rows, cols)}
+    o = np.zeros((rows, cols, 3), np.uint16)
+    cemit.build([out], 'lap_oracle_eval64')(dict(vals, laplacian=o))
+    r = eval64.evaluate([out], vals)['laplacian']
+    d = np.abs(o.astype(np.int64) - np.rint(r))
+    assert np.mean(d > 0) < 0.01 and d.max() <= 64, (np.mean(d > 0), d.max())
+
+
 def test_oracle_vs_eval64_integer_pipeline():
     """Camera pipeline (integer stages): the float64 evaluator reproduces
     the oracle exactly except where a float32-vs-float64 tone-curve entry
