@@ -23,7 +23,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 @pytest.mark.parametrize("R,C,tiles,opts", [(37, 101, [16, 32], None), (62, 70, [32, 64], {'block': (2, 4)}),
                                             (9, 13, [4, 8], {'block': (4, 4)}),
-                                            (61, 75, [8, 32], {'roll': 3})])
+                                            (61, 75, [8, 32], {'roll': 3}),
+                                            (53, 97, [16, 64], {'no_offset_inline': 1, 'threads': 512})])
 def test_emulated_harris(R, C, tiles, opts):
     pd = {}
     out = harris_pipe(pd)

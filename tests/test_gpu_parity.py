@@ -39,6 +39,15 @@ def test_harris_small_bitexact(gpu, R, C, tiles):
     assert bits_equal(g, o)          # defined region and untouched border
 
 
+@pytest.mark.parametrize("R,C,tiles,threads", [(253, 301, [32, 64], 256), (130, 517, [16, 512], 512),
+                                               (2046, 2046, [16, 512], 512)])
+def test_harris_stored_products_bitexact(gpu, R, C, tiles, threads):
+    """Products stored in LDS instead of recomputed at every box-filter tap."""
+    print("case harris stored products", R, C, tiles, threads, flush=True)
+    g, o = _harris(R, C, tiles, gpu_options={'no_offset_inline': 1, 'threads': threads})
+    assert bits_equal(g, o)
+
+
 def test_harris_float12_bitexact(gpu):
     g, o = _harris(200, 180, [32, 64], mode='float12')
     assert bits_equal(g, o)
