@@ -203,7 +203,10 @@ class Pipeline(object):
         for g in self._groups:
             g.comps[0].group = g
         self._link_groups()
-        self.auto_group()
+        if self._grouping:
+            self.manual_group()
+        else:
+            self.auto_group()
         self._order_groups()
 
         # liveness of live-outs over the group schedule (liveness.py:57-80)
@@ -612,6 +615,43 @@ class Pipeline(object):
                     merged = self.merge_groups(merged, ch)
                 changed = True
                 break
+
+    def manual_group(self):
+        """The caller's grouping (``buildPipeline(grouping=...)``, lists of
+        Functions): each list becomes one group, every other function a
+        group of its own.  The reference accepts the argument but does not
+        apply it (pipe.py:680-692, ``if self._grouping and False``) and
+        always runs auto_group."""
+        for funcs in self._grouping:
+            comps = []
+            for f in funcs:
+                cl = self._clone_map.get(f)
+                comp = self._func_map.get(cl) if cl is not None else None
+                if comp is None or comp not in self._comps:
+                    raise ValueError("grouping: %s is not a computed function of the pipeline"
+                                     % getattr(f, 'name', f))
+                comps.append(comp)
+            if len(comps) > 1 and any(c.is_reduction or c.is_const for c in comps):
+                raise ValueError("grouping: a Reduction or const function forms a group "
+                                 "of its own (%s)" % [c.func.name for c in comps])
+            g = comps[0].group
+            for c in comps[1:]:
+                if c.group is not g:
+                    g = self.merge_groups(g, c.group)
+        # the groups must run in some order: no group may feed another
+        # that feeds it back
+        indeg = dict((g, len(g.parents)) for g in self._groups)
+        ready = [g for g in self._groups if indeg[g] == 0]
+        seen = 0
+        while ready:
+            g = ready.pop()
+            seen += 1
+            for ch in g.children:
+                indeg[ch] -= 1
+                if indeg[ch] == 0:
+                    ready.append(ch)
+        if seen != len(self._groups):
+            raise ValueError("grouping: the groups depend on each other in a cycle")
 
     def _order_groups(self):
         order = level_order(self._groups, dict((g, g.parents) for g in self._groups))

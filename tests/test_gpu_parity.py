@@ -93,11 +93,12 @@ def test_unsharp_full_size(gpu):
     assert bits_equal(g, o)
 
 
-def _bilateral(rows, cols, tiles, seed=5, app=False):
+def _bilateral(rows, cols, tiles, seed=5, app=False, grouping=None):
     from polymage_amd.apps.bilateral import bilateral_grid, bilateral_grid_full
     pd = {}
     out = (bilateral_grid if app else bilateral_grid_full)(pd)
-    pipe = build(out, pd, {'rows': rows, 'cols': cols}, 'bilateral', tiles, group_size=5)
+    pipe = build(out, pd, {'rows': rows, 'cols': cols}, 'bilateral', tiles, group_size=5,
+                 grouping=grouping)
     img = inputs.smooth_field(np.random.default_rng(seed), rows, cols, 1)[0]
     g = np.full((1, rows, cols), -7.0, np.float32)
     o = np.full((1, rows, cols), -7.0, np.float32)
@@ -113,6 +114,16 @@ def test_bilateral_bitexact(gpu, rows, cols, tiles):
     print("case bilateral", rows, cols, tiles, flush=True)
     g, o = _bilateral(rows, cols, tiles)
     assert np.all(np.isfinite(o))
+    assert bits_equal(g, o)
+
+
+@pytest.mark.parametrize("rows,cols,tiles", [(100, 77, [16, 32]), (256, 192, [32, 64]),
+                                             (2560, 1536, [32, 64])])
+def test_bilateral_sliced_apart_bitexact(gpu, rows, cols, tiles):
+    """The caller's grouping: the grid blurs one kernel each, the slice
+    (interpolation and division) one kernel."""
+    print("case bilateral grouped", rows, cols, tiles, flush=True)
+    g, o = _bilateral(rows, cols, tiles, seed=1003, grouping=[['interpolated', 'filtered']])
     assert bits_equal(g, o)
 
 

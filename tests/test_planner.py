@@ -126,3 +126,30 @@ def test_halo_aligned_tiles():
     assert (16, 252) in tuner.aligned_tiles('unsharp')
     assert (16, 510) in tuner.aligned_tiles('harris')
     assert tuner.aligned_tiles('bilateral') == []
+
+
+def test_manual_grouping():
+    """buildPipeline(grouping=...) forms the listed groups (every other
+    function alone); a Reduction cannot join a group, and groups that feed
+    each other in a cycle are an error."""
+    from polymage_amd.apps.bilateral import bilateral_grid_full
+    from polymage_amd.compiler import buildPipeline
+    from polymage_amd.constructs import Condition
+    from polymage_amd.pipe import get_funcs
+
+    def pipe(names):
+        pd = {}
+        out = bilateral_grid_full(pd)
+        f = dict((x.name, x) for x in get_funcs([out])[0])
+        return buildPipeline([out], param_estimates=[(pd['rows'], 64), (pd['cols'], 48)],
+                             param_constraints=[Condition(pd['rows'], '==', 64),
+                                                Condition(pd['cols'], '==', 48)],
+                             grouping=[[f[n] for n in g] for g in names], tile_sizes=[16, 16])
+
+    p = pipe([['blurz', 'blurx'], ['interpolated', 'filtered']])
+    assert [[c.func.name for c in g.comps] for g in p._groups] == \
+        [['grid'], ['blurz', 'blurx'], ['blury'], ['interpolated', 'filtered']]
+    with pytest.raises(ValueError, match="Reduction"):
+        pipe([['grid', 'blurz']])
+    with pytest.raises(ValueError, match="cycle"):
+        pipe([['blurz', 'blury']])
