@@ -99,6 +99,15 @@ class GroupPlan(object):
         self.opts = dict(pipe.gpu_options)
         if opts:
             self.opts.update(opts)
+        if tile_sizes is None:
+            # gpu_options['group_tiles']: {function name: tile sizes} for the
+            # group holding that function (a grid-resolution group next to
+            # pixel-resolution ones); other groups use the pipeline's
+            own = self.opts.get('group_tiles') or {}
+            for c in group.comps:
+                if c.func.name in own:
+                    tile_sizes = own[c.func.name]
+                    break
         self.tile_sizes = list(tile_sizes if tile_sizes is not None else pipe.tile_sizes)
         self.env = pipe.param_env
         comps = group.comps

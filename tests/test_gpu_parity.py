@@ -93,12 +93,12 @@ def test_unsharp_full_size(gpu):
     assert bits_equal(g, o)
 
 
-def _bilateral(rows, cols, tiles, seed=5, app=False, grouping=None):
+def _bilateral(rows, cols, tiles, seed=5, app=False, grouping=None, gpu_options=None):
     from polymage_amd.apps.bilateral import bilateral_grid, bilateral_grid_full
     pd = {}
     out = (bilateral_grid if app else bilateral_grid_full)(pd)
     pipe = build(out, pd, {'rows': rows, 'cols': cols}, 'bilateral', tiles, group_size=5,
-                 grouping=grouping)
+                 grouping=grouping, gpu_options=gpu_options)
     img = inputs.smooth_field(np.random.default_rng(seed), rows, cols, 1)[0]
     g = np.full((1, rows, cols), -7.0, np.float32)
     o = np.full((1, rows, cols), -7.0, np.float32)
@@ -117,13 +117,19 @@ def test_bilateral_bitexact(gpu, rows, cols, tiles):
     assert bits_equal(g, o)
 
 
-@pytest.mark.parametrize("rows,cols,tiles", [(100, 77, [16, 32]), (256, 192, [32, 64]),
-                                             (2560, 1536, [32, 64])])
-def test_bilateral_sliced_apart_bitexact(gpu, rows, cols, tiles):
+@pytest.mark.parametrize("rows,cols,tiles,grid_tiles", [(100, 77, [16, 32], None), (256, 192, [32, 64], None),
+                                                        (2560, 1536, [32, 64], None),
+                                                        (100, 77, [16, 32], [4, 8]),
+                                                        (2560, 1536, [32, 64], [8, 8])])
+def test_bilateral_sliced_apart_bitexact(gpu, rows, cols, tiles, grid_tiles):
     """The caller's grouping: the grid blurs one kernel each, the slice
-    (interpolation and division) one kernel."""
-    print("case bilateral grouped", rows, cols, tiles, flush=True)
-    g, o = _bilateral(rows, cols, tiles, seed=1003, grouping=[['interpolated', 'filtered']])
+    (interpolation and division) one kernel; with ``grid_tiles`` the blur
+    kernels get tile sizes of their own (gpu_options['group_tiles'])."""
+    print("case bilateral grouped", rows, cols, tiles, grid_tiles, flush=True)
+    opts = {'group_tiles': dict((n, grid_tiles) for n in ('blurz', 'blurx', 'blury'))} \
+        if grid_tiles else None
+    g, o = _bilateral(rows, cols, tiles, seed=1003, grouping=[['interpolated', 'filtered']],
+                      gpu_options=opts)
     assert bits_equal(g, o)
 
 
