@@ -27,11 +27,12 @@ class Workload(object):
 
 
 def _pipe(out, pd, sizes, name, tiles, group_size, literal_mode='double', gpu_options=None,
-          extra_est=()):
+          extra_est=(), inline=()):
     est = [(pd[k], v) for k, v in sizes.items()] + list(extra_est)
     cons = [Condition(pd[k], '==', v) for k, v in sizes.items()]
     return buildPipeline([out], param_estimates=est, param_constraints=cons,
                          tile_sizes=tiles, group_size=group_size, pipe_name=name,
+                         inline_directives=list(inline),
                          literal_mode=literal_mode, gpu_options=gpu_options)
 
 
@@ -123,17 +124,25 @@ def campipe(rows=1216, cols=1664, tiles=(16, 64), gpu_options=None, with_data=Tr
 
 
 def interpolate(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
-                group_size=5, literal_mode='double'):
+                group_size=5, literal_mode='double', inline_level0=True):
     """Multiscale interpolation (10 pyramid levels) of an RGBA float image
     with a one-pixel ghost border; RGB float out.  Colour and alpha are
     smooth random fields, alpha kept in [0.05, 1] so the normalisation
-    never divides by zero."""
+    never divides by zero.
+
+    ``inline_level0``: the premultiplied level 0 (``img(c) * img(3)``, a
+    float product) is inlined into its two readers (the first downsampling
+    and the level-0 blend), the reference's ``inline_directives``
+    (inline.py:29-199); otherwise auto_group leaves it a kernel of its own
+    that writes 4 full-resolution planes for two later kernels to read."""
     from .interpolate import interpolate as interp
+    from ..pipe import get_funcs
     pd = {}
     out = interp(pd)
     R, C = rows, cols
+    inline = [f for f in get_funcs([out])[0] if f.name == 'downsampled_L0'] if inline_level0 else []
     pipe = _pipe(out, pd, {'R': R, 'C': C}, 'interpolate', list(tiles), group_size,
-                 gpu_options=gpu_options, literal_mode=literal_mode)
+                 gpu_options=gpu_options, literal_mode=literal_mode, inline=inline)
     vals = {'R': R, 'C': C}
     if with_data:
         g = inputs.rng(6)

@@ -8,17 +8,17 @@ from polymage_amd.constructs import Condition
 
 
 def build(out, pd, sizes, name, tile_sizes, group_size=11, literal_mode='double',
-          gpu_options=None, params=None, grouping=None):
-    """``grouping``: lists of function names, resolved in ``out``'s DAG."""
+          gpu_options=None, params=None, grouping=None, inline=None):
+    """``grouping`` (lists of function names) and ``inline`` (function
+    names) are resolved in ``out``'s DAG."""
     est = [(pd[k], v) for k, v in sizes.items()]
     cons = [Condition(pd[k], '==', v) for k, v in sizes.items()]
-    groups = []
-    if grouping:
-        from polymage_amd.pipe import get_funcs
-        funcs = dict((f.name, f) for f in get_funcs([out])[0])
-        groups = [[funcs[n] for n in g] for g in grouping]
+    from polymage_amd.pipe import get_funcs
+    funcs = dict((f.name, f) for f in get_funcs([out])[0])
+    groups = [[funcs[n] for n in g] for g in (grouping or [])]
     return buildPipeline([out], param_estimates=est, param_constraints=cons,
                          grouping=groups, tile_sizes=tile_sizes, group_size=group_size,
+                         inline_directives=[funcs[n] for n in (inline or [])],
                          pipe_name=name, literal_mode=literal_mode, gpu_options=gpu_options)
 
 

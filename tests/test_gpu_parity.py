@@ -205,11 +205,11 @@ def _campipe_vals(img):
             'matrix_7000': np.array(MATRIX_7000, np.float32)}
 
 
-def _interpolate(R, C, tiles, gs=5, seed=11):
+def _interpolate(R, C, tiles, gs=5, seed=11, inline=None):
     from polymage_amd.apps.interpolate import interpolate
     pd = {}
     out = interpolate(pd)
-    pipe = build(out, pd, {'R': R, 'C': C}, 'interpolate', tiles, group_size=gs)
+    pipe = build(out, pd, {'R': R, 'C': C}, 'interpolate', tiles, group_size=gs, inline=inline)
     img = inputs.smooth_field(np.random.default_rng(seed), R + 2, C + 2, 4)
     img[3] = 0.05 + 0.95 * img[3]
     img = np.ascontiguousarray(img, np.float32)
@@ -225,6 +225,16 @@ def _interpolate(R, C, tiles, gs=5, seed=11):
 def test_interpolate_bitexact(gpu, R, C, tiles, gs):
     print("case interpolate", R, C, tiles, gs, flush=True)
     g, o = _interpolate(R, C, tiles, gs)
+    assert bits_equal(g, o)
+
+
+@pytest.mark.parametrize("R,C,tiles,gs", [(70, 90, [16, 32], 5), (301, 517, [8, 64], 5),
+                                          (2560, 1536, [8, 64], 5)])
+def test_interpolate_inlined_level0_bitexact(gpu, R, C, tiles, gs):
+    """The premultiplied level 0 inlined into its two readers (the
+    benchmark schedule) against the oracle of the pipeline as written."""
+    print("case interpolate inlined level 0", R, C, tiles, gs, flush=True)
+    g, o = _interpolate(R, C, tiles, gs, seed=1006, inline=['downsampled_L0'])
     assert bits_equal(g, o)
 
 
