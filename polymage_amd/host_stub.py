@@ -115,7 +115,6 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
     w('static void* pmx_fn[64][%d];' % max(1, nk))
     w('static void* pmx_fn_batch[64][%d];' % max(1, nk))
     w('static void* pmx_zero_fn[64];')
-    w('static char pmx_slot_buf[256];')
     w('static int pmx_load(int dev) {')
     w('  if (dev < 0 || dev >= 64) { pmx_set_error("device index"); return -1; }')
     w('  if (pmx_mod[dev]) return 0;')
@@ -267,19 +266,30 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     w('  void* frames[%d] = {};' % max(1, nk))
     w('  long long total[%d] = {};' % max(1, nk))
     w('  void* counters = nullptr; void* graph = nullptr;')
+    w('  std::vector<void*> owned;  // per-frame intermediates of this handle')
     w('  long long grid[%d] = {};' % max(1, nk))
     for p in params:
         w('  std::vector<%s> p_%s;' % (c_type(p.typ), p.name))
     for f in inputs + outputs:
         w('  std::vector<void*> io_%s;' % f.name)
     w('};')
+    # one release path for destroy and for every failure inside create
+    w('static int pmx_batch_%s_free(pmx_batch_%s* B) {' % (name, name))
+    w('  if (!B) return 0;')
+    w('  if (B->graph) pmx_graph_destroy(B->graph);')
+    w('  for (int k = 0; k < %d; ++k) if (B->frames[k]) pmx_free(B->frames[k]);' % max(1, nk))
+    w('  if (B->counters) pmx_free(B->counters);')
+    w('  for (void* p : B->owned) pmx_free(p);')
+    w('  delete B;')
+    w('  return 0;')
+    w('}')
     sig = ['int n'] + ['const %s* %s_arr' % (c_type(p.typ), p.name) for p in params] + \
         ['void* const* %s_arr' % f.name for f in inputs + outputs] + ['void** handle']
     w('extern "C" int pipeline_%s_batch_create(%s) {' % (name, ', '.join(sig)))
     w('  if (n <= 0) { pmx_set_error("batch: no frames"); return -22; }')
     w('  pmx_batch_%s* B = new pmx_batch_%s(); B->n = n;' % (name, name))
-    w('  if (hipGetDevice(&B->dev) != hipSuccess) { delete B; return -1; }')
-    w('  int rc = pmx_load(B->dev); if (rc) { delete B; return rc; }')
+    w('  if (hipGetDevice(&B->dev) != hipSuccess) { pmx_batch_%s_free(B); return -1; }' % name)
+    w('  int rc = pmx_load(B->dev); if (rc) { pmx_batch_%s_free(B); return rc; }' % name)
     for p in params:
         w('  B->p_%s.assign(%s_arr, %s_arr + n);' % (p.name, p.name, p.name))
     for f in inputs + outputs:
@@ -290,8 +300,7 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     for p in params:
         w('    const %s %s = %s_arr[i];' % (c_type(p.typ), p.name, p.name))
     for l in _validation_lines(pipe, pr):
-        w('    ' + l.replace('return -22;', 'delete B; return -22;'))
-    w('    char slot[256];')
+        w('    ' + l.replace('return -22;', 'pmx_batch_%s_free(B); return -22;' % name))
     bufname = {}
     for f in inputs + outputs:
         bufname[f] = '%s_arr[i]' % f.name
@@ -299,8 +308,8 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
         f = cp.func
         b = 'b_%s' % f.name
         w('    void* %s = nullptr;' % b)
-        w('    rc = %s(%s, nullptr, &%s); if (rc) { delete B; return rc; }' % (
-            cp.upload_fn, ', '.join(p.name for p in cp.params), b))
+        w('    rc = %s(%s, nullptr, &%s); if (rc) { pmx_batch_%s_free(B); return rc; }' % (
+            cp.upload_fn, ', '.join(p.name for p in cp.params), b, name))
         bufname[f] = b
     for k in kernels:
         for f in k.buffers:
@@ -308,11 +317,11 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
                 continue
             b = 'b_%s' % f.name
             w('    void* %s = nullptr;' % b)
-            w('    snprintf(slot, sizeof slot, "%s/batch/%%d/%s", i);' % (name, f.name))
-            w('    rc = pmx_arena_get(slot, (size_t)(%s) * sizeof(%s), &%s); if (rc) { delete B; return rc; }'
-              % (_size(f, pr), c_type(f.typ), b))
+            w('    rc = pmx_malloc(&%s, (size_t)(%s) * sizeof(%s)); if (rc) { pmx_batch_%s_free(B); return rc; }'
+              % (b, _size(f, pr), c_type(f.typ), name))
+            w('    B->owned.push_back(%s);' % b)
             bufname[f] = b
-    w('    void* pmx_err = nullptr; rc = pmx_arena_get_zeroed("%s/err", 64, &pmx_err); if (rc) { delete B; return rc; }' % name)
+    w('    void* pmx_err = nullptr; rc = pmx_arena_get_zeroed("%s/err", 64, &pmx_err); if (rc) { pmx_batch_%s_free(B); return rc; }' % (name, name))
     for k in kernels:
         r = 'rec_%s[i]' % k.name
         w('    %s.pmx_err = (int*)pmx_err;' % r)
@@ -328,20 +337,19 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     for ki, k in enumerate(kernels):
         w('  { long long acc = 0; for (int i = 0; i < n; ++i) { long long c = rec_%s[i].pmx_tile0; '
           'rec_%s[i].pmx_tile0 = acc; acc += c; } B->total[%d] = acc; }' % (k.name, k.name, ki))
-        w('  snprintf(pmx_slot_buf, sizeof pmx_slot_buf, "%s/batch/rec%d/%%p", (void*)B);' % (name, ki))
-        w('  rc = pmx_malloc(&B->frames[%d], sizeof(%s_frame) * (size_t)n); if (rc) { delete B; return rc; }'
-          % (ki, k.name))
+        w('  rc = pmx_malloc(&B->frames[%d], sizeof(%s_frame) * (size_t)n); if (rc) { pmx_batch_%s_free(B); return rc; }'
+          % (ki, k.name, name))
         w('  rc = pmx_memcpy_h2d(B->frames[%d], rec_%s.data(), sizeof(%s_frame) * (size_t)n, nullptr); '
-          'if (rc) { delete B; return rc; }' % (ki, k.name, k.name))
+          'if (rc) { pmx_batch_%s_free(B); return rc; }' % (ki, k.name, k.name, name))
     # persistent grids: as many workgroups as stay resident on the device
-    w('  { int cus = 0; rc = pmx_device_cus(B->dev, &cus); if (rc) { delete B; return rc; }')
+    w('  { int cus = 0; rc = pmx_device_cus(B->dev, &cus); if (rc) { pmx_batch_%s_free(B); return rc; }' % name)
     for ki, k in enumerate(kernels):
         w('    { int occ = 0; rc = pmx_occupancy(pmx_fn_batch[B->dev][%d], %d, 0, &occ); '
-          'if (rc) { delete B; return rc; } if (occ < 1) occ = 1; B->grid[%d] = (long long)occ * cus; }'
-          % (ki, k.threads, ki))
+          'if (rc) { pmx_batch_%s_free(B); return rc; } if (occ < 1) occ = 1; B->grid[%d] = (long long)occ * cus; }'
+          % (ki, k.threads, name, ki))
     w('  }')
-    w('  rc = pmx_malloc(&B->counters, 64 * %d); if (rc) { delete B; return rc; }' % max(1, nk))
-    w('  rc = pmx_stream_sync(nullptr); if (rc) { delete B; return rc; }')
+    w('  rc = pmx_malloc(&B->counters, 64 * %d); if (rc) { pmx_batch_%s_free(B); return rc; }' % (max(1, nk), name))
+    w('  rc = pmx_stream_sync(nullptr); if (rc) { pmx_batch_%s_free(B); return rc; }' % name)
     w('  *handle = (void*)B;')
     w('  return 0;')
     w('}')
@@ -383,12 +391,6 @@ def _emit_batch_api(pipe, kernels, host_consts, L, pr, bufname_fn=None):
     w('  return 0;')
     w('}')
     w('extern "C" int pipeline_%s_batch_destroy(void* handle) {' % name)
-    w('  pmx_batch_%s* B = (pmx_batch_%s*)handle;' % (name, name))
-    w('  if (!B) return 0;')
-    w('  if (B->graph) pmx_graph_destroy(B->graph);')
-    w('  for (int k = 0; k < %d; ++k) if (B->frames[k]) pmx_free(B->frames[k]);' % max(1, nk))
-    w('  if (B->counters) pmx_free(B->counters);')
-    w('  delete B;')
-    w('  return 0;')
+    w('  return pmx_batch_%s_free((pmx_batch_%s*)handle);' % (name, name))
     w('}')
 

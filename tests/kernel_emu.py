@@ -25,7 +25,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 EMU = os.path.join(HERE, 'emu')
-CACHE = os.environ.get('PMX_EMU_CACHE', os.path.join(REPO, '_work', 'emu_cache'))
+CACHE = os.environ.get('PMX_EMU_CACHE', os.path.join(REPO, 'tests', '.emu_cache'))
 
 _SIG = re.compile(r'extern "C" __global__ (?:__launch_bounds__\(\d+\) )?void (\w+)\(([^)]*)\)')
 
@@ -128,8 +128,9 @@ class EmulatedPipeline(object):
         fn.restype = ctypes.c_int
         self._check(fn(*args), 'pipeline_%s_host' % self.name)
 
-    def run_batch(self, frames, mode='persistent'):
-        """frames: list of value dicts with host arrays; runs the batch API."""
+    def batch_create(self, frames):
+        """frames: list of value dicts with host arrays; returns the handle
+        of pipeline_<name>_batch_create."""
         from polymage_amd.utils import _CTYPE
         n = len(frames)
         args = [ctypes.c_int(n)]
@@ -146,6 +147,17 @@ class EmulatedPipeline(object):
         h = ctypes.c_void_p()
         args.append(ctypes.byref(h))
         self._check(self.lib['pipeline_%s_batch_create' % self.name](*args), 'batch_create')
+        return h
+
+    def batch_run(self, h, mode='persistent'):
         fn = {'persistent': 'batch_run', 'frames': 'batch_run_frames'}[mode]
         self._check(self.lib['pipeline_%s_%s' % (self.name, fn)](h, ctypes.c_void_p(0)), fn)
+
+    def batch_destroy(self, h):
         self._check(self.lib['pipeline_%s_batch_destroy' % self.name](h), 'batch_destroy')
+
+    def run_batch(self, frames, mode='persistent'):
+        """frames: list of value dicts with host arrays; runs the batch API."""
+        h = self.batch_create(frames)
+        self.batch_run(h, mode)
+        self.batch_destroy(h)

@@ -174,3 +174,64 @@ def test_emulated_pyramid_apps(app, R, C, tiles, gs):
     kernel_emu.build(pipe).run_host(dict(vals, **{oname: g}))
     cemit.build([out], app + '_oracle')(dict(vals, **{oname: o}))
     assert bits_equal(g, o)
+
+
+def _two_handles_case(asan):
+    """A small batch handle, then a larger one of the same pipeline; the
+    first handle still gives the oracle's output afterwards (each handle
+    owns its intermediates, so the second cannot free or share them)."""
+    from polymage_amd.apps.bilateral import bilateral_grid_full
+    pd = {}
+    out = bilateral_grid_full(pd)
+    pipe = build(out, pd, {'rows': 128, 'cols': 128}, 'bilateral', [16, 32], group_size=5)
+    ep = kernel_emu.build(pipe, asan=asan)
+    status = ep.lib['pipeline_bilateral_status']
+    orc = cemit.build([out], 'bilateral_oracle_full')
+
+    def frames(sizes, seed):
+        recs, refs = [], []
+        for k, (rows, cols) in enumerate(sizes):
+            img = inputs.smooth_field(np.random.default_rng(seed + k), rows, cols, 1)[0]
+            o = np.full((1, rows, cols), -7.0, np.float32)
+            orc({'rows': rows, 'cols': cols, 'input': img, 'filtered': o})
+            refs.append(o)
+            recs.append({'rows': rows, 'cols': cols, 'input': img,
+                         'filtered': np.full_like(o, -7.0)})
+        return recs, refs
+
+    small, small_ref = frames([(48, 40), (40, 56)], 11)
+    big, big_ref = frames([(112, 96), (96, 120)], 21)
+    ha = ep.batch_create(small)
+    hb = ep.batch_create(big)
+    ep.batch_run(hb)
+    for rec in small:
+        rec['filtered'][...] = -7.0
+    ep.batch_run(ha)
+    assert status() == 0
+    for rec, r in zip(small + big, small_ref + big_ref):
+        assert bits_equal(rec['filtered'], r)
+    ep.batch_destroy(hb)
+    ep.batch_run(ha)
+    assert status() == 0
+    for rec, r in zip(small, small_ref):
+        assert bits_equal(rec['filtered'], r)
+    ep.batch_destroy(ha)
+
+
+def test_emulated_batch_handles_own_intermediates():
+    _two_handles_case(asan=False)
+
+
+def test_emulated_batch_handles_under_asan():
+    """The same two-handle case with AddressSanitizer: a handle whose
+    intermediates a later handle freed would be a use-after-free."""
+    if os.environ.get('PMX_EMU_CHILD'):
+        pytest.skip("runs only as the parent's child case")
+    env = dict(os.environ, PMX_EMU_CHILD='1', ASAN_OPTIONS='detect_leaks=0',
+               LD_PRELOAD=kernel_emu.asan_runtime())
+    code = ('import sys; sys.path[:0] = [%r, %r]; import test_emulator as t; '
+            't._two_handles_case(asan=True); print("two handles ok")' % (HERE, os.path.dirname(HERE)))
+    r = subprocess.run([sys.executable, '-c', code], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, timeout=900, cwd=os.path.dirname(HERE))
+    out = r.stdout.decode(errors='replace')
+    assert r.returncode == 0 and 'two handles ok' in out, out[-3000:]
