@@ -13,10 +13,14 @@ from . import inputs
 
 
 class Workload(object):
+    """``pipe`` may be given as a zero-argument callable; the pipeline is
+    then built (buildPipeline) on first access, so the data, shapes and
+    output function of a workload are available without planning it."""
+
     def __init__(self, name, pipe, out_func, values, out_shape, mpix, ideal_bytes,
                  defined=None, dtype='f32'):
         self.name = name
-        self.pipe = pipe
+        self._pipe = pipe
         self.out_func = out_func
         self.values = values            # host arrays + scalar params
         self.out_shape = out_shape
@@ -25,15 +29,21 @@ class Workload(object):
         self.defined = defined          # slice tuple of the written region
         self.dtype = dtype
 
+    @property
+    def pipe(self):
+        if callable(self._pipe):
+            self._pipe = self._pipe()
+        return self._pipe
+
 
 def _pipe(out, pd, sizes, name, tiles, group_size, literal_mode='double', gpu_options=None,
           extra_est=(), inline=()):
     est = [(pd[k], v) for k, v in sizes.items()] + list(extra_est)
     cons = [Condition(pd[k], '==', v) for k, v in sizes.items()]
-    return buildPipeline([out], param_estimates=est, param_constraints=cons,
-                         tile_sizes=tiles, group_size=group_size, pipe_name=name,
-                         inline_directives=list(inline),
-                         literal_mode=literal_mode, gpu_options=gpu_options)
+    return lambda: buildPipeline([out], param_estimates=est, param_constraints=cons,
+                                 tile_sizes=tiles, group_size=group_size, pipe_name=name,
+                                 inline_directives=list(inline),
+                                 literal_mode=literal_mode, gpu_options=gpu_options)
 
 
 def harris(rows=2048, cols=2048, tiles=(32, 256), gpu_options=None, with_data=True,
@@ -84,17 +94,20 @@ def bilateral(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
 
 
 def laplacian(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
-              group_size=5, K=8, J=4, literal_mode='double'):
+              group_size=5, K=8, J=4, literal_mode='double', inline=()):
     """Local Laplacian, uint16 RGB, K intensity levels, J pyramid levels,
     alpha = 1/(K-1), beta = 1; the input is padded by edge replication
-    inside the pipeline so the output has the input's size."""
+    inside the pipeline so the output has the input's size.  ``inline``
+    names functions to inline into their readers (inline_directives)."""
     from .local_laplacian import local_laplacian
+    from ..pipe import get_funcs
     pd = {}
     out = local_laplacian(pd, K=K, J=J, io='u16', lut_hi_inclusive=True)
     lo, hi = pd['pad']
     R, C = rows + lo + hi, cols + lo + hi
+    funcs = [f for f in get_funcs([out])[0] if f.name in set(inline)]
     pipe = _pipe(out, pd, {'R': R, 'C': C}, 'laplacian', list(tiles), group_size,
-                 gpu_options=gpu_options, literal_mode=literal_mode)
+                 gpu_options=gpu_options, literal_mode=literal_mode, inline=funcs)
     vals = {'R': R, 'C': C, 'alpha': 1.0 / (K - 1), 'beta': 1.0}
     if with_data:
         vals['img_colour'] = inputs.laplacian_input(rows, cols)
@@ -183,16 +196,42 @@ DEFAULT_TILES = {'unsharp': (32, 256), 'harris': (32, 256), 'bilateral': (32, 64
 def prebuild_default():
     """Compile (no GPU needed) the default schedules and the schedules the
     autotuner recorded, so benchmark runs find them in the kernel cache."""
-    from .. import runtime, tuner
-    fns = {'harris': harris, 'unsharp': unsharp, 'bilateral': bilateral,
-           'laplacian': laplacian, 'campipe': campipe, 'interpolate': interpolate,
-           'blend': blend}
-    for name, fn in fns.items():
-        w = fn(with_data=False)
-        runtime.build(w.pipe)
-        for which in ('learned', 'exhaustive'):
-            rec = tuner.tuned_schedule(name, which)
-            if rec is not None:
-                w = fn(tiles=rec['tiles'], group_size=rec['group_size'],
-                       gpu_options={'threads': rec['threads']}, with_data=False)
-                runtime.build(w.pipe)
+    from .. import runtime
+    for name in ('harris', 'unsharp', 'bilateral', 'laplacian', 'campipe', 'interpolate', 'blend'):
+        for which in ('default', 'learned', 'exhaustive'):
+            runtime.build(tuned(name, which, with_data=False).pipe)
+
+
+BUILDERS = {'harris': harris, 'unsharp': unsharp, 'bilateral': bilateral,
+            'laplacian': laplacian, 'campipe': campipe, 'interpolate': interpolate,
+            'blend': blend}
+
+
+def schedule(app, which='learned'):
+    """(tiles, group_size, gpu_options, label) of ``app`` under the
+    autotuner's recorded pick ('learned' or 'exhaustive', from
+    polymage_amd/tuned.json) or the default tiles ('default', or when no
+    pick is recorded)."""
+    from .. import tuner
+    rec = tuner.tuned_schedule(app, which) if which != 'default' else None
+    if rec is None:
+        return tuple(DEFAULT_TILES[app]), None, None, 'default'
+    return tuple(rec['tiles']), rec['group_size'], {'threads': rec['threads']}, \
+        '%s: %s' % (which, rec['key'])
+
+
+def tuned(app, which='learned', **kw):
+    """The workload ``app`` built with the schedule ``schedule(app, which)``:
+    the pipeline bench.py times and the full-size parity tests check."""
+    tiles, gs, gopts, label = schedule(app, which)
+    args = dict(kw)
+    args['tiles'] = tiles
+    if gs is not None:
+        args['group_size'] = gs
+    if gopts:
+        merged = dict(gopts)
+        merged.update(args.get('gpu_options') or {})
+        args['gpu_options'] = merged
+    w = BUILDERS[app](**args)
+    w.schedule = label
+    return w

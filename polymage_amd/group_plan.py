@@ -104,10 +104,13 @@ class GroupPlan(object):
             # group holding that function (a grid-resolution group next to
             # pixel-resolution ones); other groups use the pipeline's
             own = self.opts.get('group_tiles') or {}
-            for c in group.comps:
-                if c.func.name in own:
-                    tile_sizes = own[c.func.name]
-                    break
+            picks = dict((c.func.name, tuple(own[c.func.name])) for c in group.comps
+                         if c.func.name in own)
+            if len(set(picks.values())) > 1:
+                raise ValueError("gpu_options['group_tiles']: functions of one group ask for "
+                                 "different tiles: %s" % sorted(picks.items()))
+            if picks:
+                tile_sizes = list(next(iter(picks.values())))
         self.tile_sizes = list(tile_sizes if tile_sizes is not None else pipe.tile_sizes)
         self.env = pipe.param_env
         comps = group.comps

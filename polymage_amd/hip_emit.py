@@ -21,6 +21,7 @@ reference, and ``pipeline_<name>_dev(..., stream)`` on device pointers.
 """
 
 import hashlib
+import re
 
 from .analysis import PlanError, classify_access, eval_bexpr, unconditional_refs, shift_access
 from .constructs import Reference, Image, Case, Reduction
@@ -37,6 +38,11 @@ def _prod(xs):
     return n
 
 
+# separators of the two parts in the printed source (runtime.split_source)
+HIP_MARK = '// ===== HIP kernels (hiprtc, gfx950) =====\n'
+HOST_MARK = '\n// ===== host stub (g++) =====\n'
+
+
 class GeneratedCode(object):
     """What ``Pipeline.generate_code`` returns; ``str()`` is the source
     (the HIP kernels followed by the host stub), as in pipe.py:934-958."""
@@ -49,8 +55,7 @@ class GeneratedCode(object):
         self.meta = meta
 
     def __str__(self):
-        return ('// ===== HIP kernels (hiprtc, gfx950) =====\n' + self.hip_src +
-                '\n// ===== host stub (g++) =====\n' + self.host_src)
+        return HIP_MARK + self.hip_src + HOST_MARK + self.host_src
 
     @property
     def key(self):
@@ -249,6 +254,11 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             self.d = d
             self.decls = []
             self.names = {}
+            # load forwarding between aligned sweeps: global reads at a
+            # compile-time point of the block are recorded (key -> load)
+            # or, for a later sweep, replaced by the register holding them
+            self.record = None
+            self.forward = None
 
         def _base(self, key, make):
             if key not in self.names:
@@ -322,11 +332,20 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 return 'const %s* __restrict__ %s = %s + (%s);' % (
                     c_type(f.typ), nm, _gname(f), ' + '.join(terms) if terms else '0')
             nm = self._base(('g', f.name, kinds), mk)
+            absd = []
             for j, (k, off) in enumerate(sh):
                 o = self._dsum(k, off, pr) if k != 'const' else \
                     (off if isinstance(off, int) else pr.expr(off))
+                absd.append(o)
                 parts.append((o, 'S_%s_%d' % (f.name, j) if j < nd - 1 else 1))
-            return '%s[%s]' % (nm, self._off(parts, pr))
+            load = '%s[%s]' % (nm, self._off(parts, pr))
+            if all(isinstance(o, int) for o in absd):
+                key = (f.name, kinds, tuple(absd))
+                if self.forward is not None and key in self.forward:
+                    return self.forward[key]
+                if self.record is not None:
+                    self.record.setdefault(key, load)
+            return load
 
     # a tile whose region exceeds the planned static box (possible only for
     # sizes far from the planning estimates) is skipped and reported
@@ -495,6 +514,10 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     if over2:
         w('if (%s) { if (threadIdx.x == 0) atomicOr(pmx_err, 1); return; }' % ' || '.join(over2))
 
+    fwd = _forward_pair(plan, threads)
+    fw_src = {}         # key -> load expression in the first sweep's fast path
+    fw_need = set()     # keys the later sweep's fast path reads
+    fw_vars = {}        # key -> register name
     phase_code = {}     # phase -> [(comment, nitems, lines)]
     for si, (ph, members) in enumerate(plan.sweeps):
         SL = []
@@ -502,6 +525,12 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         s0 = members[0]
         box = sweep_box[si]
         ext = s0.static_ext
+        if fwd is not None and si == fwd[1]:
+            # the later sweep walks the first one's item space (its own
+            # region checks added below), so each thread's block sits on
+            # the points it loaded there
+            box = sweep_box[fwd[0]]
+            ext = plan.sweeps[fwd[0]][1][0].static_ext
         U, V = s0.block
         nd = s0.nd
         rd, cd = s0.bdims
@@ -545,7 +574,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             w3('if (%s) continue;' % guard)
         blocked = U * V > 1
         pts = [(u, v) for u in range(U) for v in range(V)]
-        body_ind = 4 if blocked or len(members) > 1 else 3
+        body_ind = 4 if blocked or len(members) > 1 or (fwd is not None and si == fwd[1]) else 3
 
         def dvec(u, v):
             d = [0] * nd
@@ -571,6 +600,21 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             return c
 
         fitem = Item(pb, dvec(0, 0))
+        if fwd is not None and si == fwd[0]:
+            fitem.record = fw_src
+        if fwd is not None and si == fwd[1]:
+            # which of the first sweep's loads this fast path reads (a dry
+            # run of its points), then the same points reading registers
+            probe = Item(pb, dvec(0, 0))
+            probe.record = {}
+            for m in members:
+                for (u, v) in pts:
+                    probe.d = dvec(u, v)
+                    emit_point([], m, probe, body_ind, 'x', 'y')
+            for key in probe.record:
+                if key in fw_src:
+                    fw_need.add(key)
+            fitem.forward = dict((k, fw_vars[k]) for k in fw_need)
         fast = []
         for mi, m in enumerate(members):
             T = c_type(m.func.typ)
@@ -585,11 +629,21 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             for (u, v) in pts:
                 fitem.d = dvec(u, v)
                 emit_store(fast, m, fitem, body_ind, 'pv%d_%d_%d' % (mi, u, v), 'pw%d_%d_%d' % (mi, u, v))
+        if fwd is not None and si == fwd[0]:
+            # every load of this fast path a later sweep may take over, kept
+            # in a register (the point code above issued the same loads)
+            for key in sorted(fw_src):
+                fw_vars[key] = 'fw%d' % len(fw_vars)
+                fast.append('  ' * body_ind + '%s = %s;' % (fw_vars[key], fw_src[key]))
+            # the later sweep takes the registers only where this fast path
+            # ran for its item (its block inside this sweep's region)
+            fast.append('  ' * body_ind + 'fw_ok = true;')
         slow = []
         sitem = Item(pb, dvec('pu', 'pv'))
         sitem.names = fitem.names
         sitem.decls = fitem.decls
-        need_slow = blocked or len(members) > 1
+        # (a sweep walking another's item space checks its own region)
+        need_slow = blocked or len(members) > 1 or (fwd is not None and si == fwd[1])
         if need_slow:
             slow.append('  ' * 4 + 'for (int pu = 0; pu < %d; ++pu) for (int pv = 0; pv < %d; ++pv) {' % (U, V))
             pt = ['b%d%s' % (i, ' + pu' if i == rd else (' + pv' if i == cd else '')) for i in range(nd)]
@@ -613,6 +667,8 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 for c in inside(m, lo_t, hi_t):
                     if c not in fc:
                         fc.append(c)
+            if fwd is not None and si == fwd[1] and fw_need:
+                fc.append('fw_ok')
             w3('if (%s) {' % (' && '.join(fc) if fc else 'true'))
             SL.extend(fast)
             w3('} else {')
@@ -622,6 +678,19 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             SL.extend(fast)
         phase_code.setdefault(ph, []).append((comment, nitems, SL))
 
+    if fwd is not None:
+        # registers carried from the first sweep to the later one (each
+        # thread runs one item of each, on the same block)
+        ftyp = dict((f.name, c_type(f.typ)) for f in buffers)
+        keep = set(fw_vars[k] for k in fw_need)
+        w('bool fw_ok = false; (void)fw_ok;')
+        for key in sorted(fw_vars, key=lambda k: fw_vars[k]):
+            if fw_vars[key] in keep:
+                w('%s %s = (%s)0;' % (ftyp[key[0]], fw_vars[key], ftyp[key[0]]))
+        for ph, sws in phase_code.items():
+            for (_, _, SL) in sws:
+                SL[:] = [l for l in SL if not (_FW_LINE.match(l.strip()) and
+                                               l.strip().split(' ', 1)[0] not in keep)]
     # one loop per phase over the concatenated item spaces of its sweeps, so
     # several small sweeps share the workgroup's lanes instead of each
     # leaving most of them idle in its last iteration
@@ -659,6 +728,65 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     return wrap_kernel(kname, params, buffers, writes, threads, plan.lds_bytes, L, roll=roll), info
 
 
+
+
+_FW_LINE = re.compile(r'^fw\d+ = ')
+
+
+def _forward_pair(plan, threads):
+    """(first, later) sweep indices whose global loads can be forwarded in
+    registers, or None.  Both are the only sweep of their phase, have the
+    same block shape and dimensions, need one item per thread (so each
+    thread's registers from the first survive the barrier into the
+    later), and the later one's region lies inside the first's on every
+    sampled tile (it then walks the first's item space).  Unsharp: the
+    mask reads the image at the points the blur loaded; without this the
+    second read goes back to L2."""
+    import numpy as np
+    if plan.opts.get('no_forward'):
+        return None
+    by_phase = {}
+    for si, (ph, members) in enumerate(plan.sweeps):
+        by_phase.setdefault(ph, []).append(si)
+
+    def items(s):
+        U, V = s.block
+        rd, cd = s.bdims
+        cnt = list(s.static_ext)
+        if cd is not None:
+            cnt[cd] = -(-cnt[cd] // V)
+        if rd is not None:
+            cnt[rd] = -(-cnt[rd] // U)
+        return _prod(cnt)
+
+    phs = sorted(by_phase)
+    for a_i, a in enumerate(phs):
+        if len(by_phase[a]) != 1:
+            continue
+        s1i = by_phase[a][0]
+        m1 = plan.sweeps[s1i][1]
+        if len(m1) != 1 or items(m1[0]) > threads:
+            continue
+        s1 = m1[0]
+        for b in phs[a_i + 1:]:
+            if len(by_phase[b]) != 1:
+                continue
+            s2i = by_phase[b][0]
+            m2 = plan.sweeps[s2i][1]
+            if len(m2) != 1:
+                continue
+            s2 = m2[0]
+            if (s2.nd, s2.block, s2.bdims) != (s1.nd, s1.block, s1.bdims):
+                continue
+            g1 = set(r.objectRef for (t, r) in plan.closure_refs(s1) if not plan.in_group(r.objectRef))
+            g2 = set(r.objectRef for (t, r) in plan.closure_refs(s2) if not plan.in_group(r.objectRef))
+            if not (g1 & g2):
+                continue
+            inside = all(np.all(l2 >= l1) and np.all(h2 <= h1)
+                         for (l1, h1), (l2, h2) in zip(plan._region_samples[s1], plan._region_samples[s2]))
+            if inside and all(e2 <= e1 for e1, e2 in zip(s1.static_ext, s2.static_ext)):
+                return (s1i, s2i)
+    return None
 
 
 def _magic24(n, d):
@@ -841,7 +969,10 @@ def emit_pipeline(pipe, is_extern_c_func=True, are_io_void_ptrs=True):
         if isinstance(plan, GroupPlan):
             src, info = emit_group_kernel(pipe, plan, kname, threads, mode, xcd)
         elif isinstance(plan, ReductionPlan):
-            src, info = emit_reduction_kernel(pipe, plan, kname, mode)
+            for src, info in emit_reduction_kernel(pipe, plan, kname, mode):
+                srcs.append(src)
+                kernels.append(info)
+            continue
         else:
             if plan.on_host:
                 host_consts.append(plan)

@@ -545,16 +545,19 @@ class Pipeline(object):
         return merged
 
     def _fusible(self, comps):
-        """GPU-side fusion rules on top of the reference's (grouping.py):
-        (1) an in-group reference whose argument is data dependent must hit a
-        producer dimension small enough to keep whole in the tile (<= 64
-        points); (2) in-group references must not read a producer through
-        a non-monotone map on a dimension longer than that."""
+        return self._unfusible_reason(comps) is None
+
+    def _unfusible_reason(self, comps):
+        """GPU-side fusion rule on top of the reference's (grouping.py): an
+        in-group reference whose argument is data dependent (or not an
+        affine map of the reader's variables) must hit a producer dimension
+        small enough to keep whole in the tile (<= 64 points).  Returns
+        None when the functions can form one group, else the reason."""
         cset = set(comps)
         for c in comps:
             f = c.func
             if c.is_reduction:
-                return False
+                return '%s is a Reduction' % f.name
             for part in c.parts:
                 refs = part.expr.collect(Reference)
                 for cond in part.pred:
@@ -569,8 +572,10 @@ class Pipeline(object):
                         if acc.kind == '*':
                             lo, hi = pdom[i]
                             if hi - lo + 1 > 64:
-                                return False
-        return True
+                                return ('%s reads %s at a data-dependent index along a dimension '
+                                        'of %d points (at most 64 fit a tile)'
+                                        % (f.name, r.objectRef.name, hi - lo + 1))
+        return None
 
     def auto_group(self):
         """Greedy level-by-level fusion (grouping.py:56-239)."""
@@ -634,6 +639,11 @@ class Pipeline(object):
             if len(comps) > 1 and any(c.is_reduction or c.is_const for c in comps):
                 raise ValueError("grouping: a Reduction or const function forms a group "
                                  "of its own (%s)" % [c.func.name for c in comps])
+            if len(comps) > 1:
+                why = self._unfusible_reason(comps)
+                if why:
+                    raise ValueError("grouping: %s cannot form one group: %s"
+                                     % ([c.func.name for c in comps], why))
             g = comps[0].group
             for c in comps[1:]:
                 if c.group is not g:

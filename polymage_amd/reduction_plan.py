@@ -14,8 +14,17 @@
   pre-image of the reduction domain in the reference's lexicographic order
   and accumulates every data-dependent dimension (``2 + (int)(img*10)``) in
   registers, so the float sums are the reference's serial sums, bit for
-  bit, with no atomics.  Other reductions run the reference's serial loop
-  in a single thread (exact, slow; histograms and other small cases).
+  bit, with no atomics.  Reduction variables that appear in no target
+  index (a row reduction ``hist(c, x) op= img(x, y)`` over y) are looped
+  inside each owner thread in the domain's order, so those sums keep the
+  reference's order too.  Integer reductions whose target is data
+  dependent (histograms: ``hist(bin(img(x, y))) += 1``) have a result that
+  does not depend on the order of their updates (integer +, *, min and max
+  are associative and commutative, modulo 2^32 for + and *), so they run
+  one thread per reduction point with device atomics, after a kernel that
+  writes the default value.  The remaining cases (a float reduction into
+  data-dependent cells with no owned dimension) run the reference's serial
+  loop in a single thread (exact, slow).
 """
 
 from fractions import Fraction
@@ -64,7 +73,10 @@ class ConstPlan(object):
         return 'group %d: const table %s (%s)' % (self.gindex, self.func.name,
                                                   'host' if self.on_host else 'device')
 
-    def host_code(self, pr):
+    def host_code(self, pr, host_only=False):
+        """C++ computing the table on the host (``<upload_fn>_host``) and,
+        unless ``host_only``, uploading it to the device arena once per
+        parameter key (``<upload_fn>``)."""
         f = self.func
         T = c_type(f.typ)
         nd = len(f.variables)
@@ -73,6 +85,9 @@ class ConstPlan(object):
         hp = Printer(vn, None, self.pipe.literal_mode)
         L = []
         w = L.append
+        # host versions of the math helpers, once per translation unit
+        w('#ifndef PMX_HOST_MATH')
+        w('#define PMX_HOST_MATH')
         w('static inline float pmx_exp(float v) { return std::exp(v); }')
         w('static inline double pmx_exp(double v) { return std::exp(v); }')
         w('static inline double pmx_exp(int v) { return std::exp((double)v); }')
@@ -85,12 +100,16 @@ class ConstPlan(object):
         w('static inline double pmx_sin(double v) { return std::sin(v); }')
         w('static inline float pmx_cos(float v) { return std::cos(v); }')
         w('static inline double pmx_cos(double v) { return std::cos(v); }')
+        w('#endif')
         sig = ', '.join('%s %s' % (c_type(p.typ), p.name) for p in self.params)
         used = set(id(q) for q in f.getObjects(Parameter))
         keyp = [p for p in self.params if id(p) in used]
         w('static std::map<std::vector<unsigned char>, std::vector<%s> > pmx_tab_%s;' % (T, f.name))
         w('static std::set<std::pair<int, std::vector<unsigned char> > > pmx_tab_up_%s;' % f.name)
-        w('static int %s(%s%svoid* stream, void** out) {' % (self.upload_fn, sig, ', ' if sig else ''))
+        # the host table (computed once per parameter key, kept for the
+        # process) and its byte size
+        w('static int %s_host(%s%sconst %s** data, size_t* bytes, std::vector<unsigned char>* keyp) {'
+          % (self.upload_fn, sig, ', ' if sig else '', T))
         for p in self.params:
             w('  (void)%s;' % p.name)
         w('  std::vector<unsigned char> key;')
@@ -129,13 +148,23 @@ class ConstPlan(object):
         for i in reversed(range(nd)):
             w('    ' + '  ' * i + '}')
         w('  }')
+        w('  *data = tab.data(); *bytes = (size_t)n * sizeof(%s);' % T)
+        w('  if (keyp) *keyp = key;')
+        w('  return 0;')
+        w('}')
+        if host_only:
+            return L
+        w('static int %s(%s%svoid* stream, void** out) {' % (self.upload_fn, sig, ', ' if sig else ''))
+        w('  const %s* tdata = nullptr; size_t tbytes = 0; std::vector<unsigned char> key;' % T)
+        w('  int rc = %s_host(%s%s&tdata, &tbytes, &key); if (rc) return rc;'
+          % (self.upload_fn, ', '.join(p.name for p in self.params), ', ' if self.params else ''))
         w('  int dev = 0; if (hipGetDevice(&dev) != hipSuccess) return -1;')
         w('  unsigned long long h = 1469598103934665603ULL;')
         w('  for (unsigned char b : key) { h ^= b; h *= 1099511628211ULL; }')
         w('  char slot[128]; snprintf(slot, sizeof slot, "%s/const/%s/%%016llx", h);' % (self.pipe.name, f.name))
-        w('  int rc = pmx_arena_get(slot, (size_t)n * sizeof(%s), out); if (rc) return rc;' % T)
+        w('  rc = pmx_arena_get(slot, tbytes, out); if (rc) return rc;')
         w('  if (!pmx_tab_up_%s.count(std::make_pair(dev, key))) {' % f.name)
-        w('    rc = pmx_memcpy_h2d(*out, tab.data(), (size_t)n * sizeof(%s), stream); if (rc) return rc;' % T)
+        w('    rc = pmx_memcpy_h2d(*out, tdata, tbytes, stream); if (rc) return rc;')
         w('    rc = pmx_stream_sync(stream); if (rc) return rc;')
         w('    pmx_tab_up_%s.insert(std::make_pair(dev, key));' % f.name)
         w('  }')
@@ -247,13 +276,25 @@ class ReductionPlan(object):
         owned = {}
         self.pre_size = {}
         ok = True
+        self.loop_vars = []
         for j, rv in enumerate(self.rvars):
             dim = None
+            hits = []
             for rc in self.cases:
                 tgt = rc.red.accumulate_ref.arguments
                 hit = [d for d in range(nd)
                        if classify_access(tgt[d], self.rvars).kind == 'var'
                        and classify_access(tgt[d], self.rvars).var is rv]
+                hits.append(hit)
+            if all(not h for h in hits) and not any(
+                    rv in [v for v in _vars_of(rc.red.accumulate_ref.arguments[d])]
+                    for rc in self.cases for d in range(nd)
+                    if classify_access(rc.red.accumulate_ref.arguments[d], self.rvars).kind != '*'):
+                # in no affine target index: every owner thread walks it
+                self.loop_vars.append(rv)
+                continue
+            for rc, hit in zip(self.cases, hits):
+                tgt = rc.red.accumulate_ref.arguments
                 if len(hit) != 1:
                     ok = False
                     break
@@ -281,12 +322,37 @@ class ReductionPlan(object):
             hi = ceval_int(f.domain[d].upperBound, env)
             nacc *= (hi - lo + 1)
         self.nacc = nacc
-        self.owner = ok and len(owned) == len(self.rvars) and nacc <= self.MAX_ACC
+        self.owner = ok and bool(owned) and len(owned) + len(self.loop_vars) == len(self.rvars) \
+            and nacc <= self.MAX_ACC
         self.owned = owned
+        self.atomic = (not self.owner) and self._atomic_ok()
+
+    def _atomic_ok(self):
+        """Integer target, integer values of the target's promoted type and
+        an operator whose result is independent of the update order."""
+        from .expression import getType
+        from .expr_types import Int, UInt, Char, UChar, Short, UShort
+        f = self.func
+        if f.typ not in (Int, UInt):
+            return False
+        for rc in self.cases:
+            if rc.red.op_type not in (Op.Sum, Op.Mul, Op.Min, Op.Max):
+                return False
+            vt = getType(rc.red.expression)
+            if vt in (Char, UChar, Short, UShort):
+                vt = Int
+            if vt is not f.typ:
+                return False
+        return True
 
     def describe(self):
+        kind = 'owner cells' if self.owner else ('atomic' if self.atomic else 'serial')
         return 'group %d: reduction %s (%s, %d accumulators per cell)' % (
-            self.gindex, self.func.name, 'owner cells' if self.owner else 'serial', self.nacc)
+            self.gindex, self.func.name, kind, self.nacc)
+
+
+def _vars_of(e):
+    return e.collect(Variable) if hasattr(e, 'collect') else []
 
 
 def _prod_int(xs):
@@ -344,9 +410,11 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
     exts = [_extent_c(f, d, base) for d in range(nd)]
     lbs = [base.expr(f.domain[d].lowerBound) for d in range(nd)]
 
-    def red_body(ind, target_ok):
+    def red_body(ind, target_ok, update=None):
         """Statements processing the current reduction point (names
-        r0..rk bound).  ``target_ok(dimvals)`` filters owned cells."""
+        r0..rk bound).  ``target_ok(dimvals)`` filters owned cells;
+        ``update(op, val)`` gives the statement applying one value (by
+        default the C update of ``acc_slot()``)."""
         pr = Printer(dict(zip(plan.rvars, rnames)), cb, mode, lazy=True)
         for rc in plan.cases:
             red = rc.red
@@ -379,16 +447,19 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
             extra = target_ok()
             if extra:
                 conds.append(extra)
-            slot = acc_slot()
             op = red.op_type
-            if op == Op.Sum:
-                upd = '%s = (%s + %s);' % (slot, slot, val)
-            elif op == Op.Mul:
-                upd = '%s = (%s * %s);' % (slot, slot, val)
-            elif op == Op.Min:
-                upd = '%s = ((%s < %s)? (%s)(%s): %s);' % (slot, val, slot, T, val, slot)
+            if update is not None:
+                upd = update(op, val)
             else:
-                upd = '%s = ((%s > %s)? (%s)(%s): %s);' % (slot, val, slot, T, val, slot)
+                slot = acc_slot()
+                if op == Op.Sum:
+                    upd = '%s = (%s + %s);' % (slot, slot, val)
+                elif op == Op.Mul:
+                    upd = '%s = (%s * %s);' % (slot, slot, val)
+                elif op == Op.Min:
+                    upd = '%s = ((%s < %s)? (%s)(%s): %s);' % (slot, val, slot, T, val, slot)
+                else:
+                    upd = '%s = ((%s > %s)? (%s)(%s): %s);' % (slot, val, slot, T, val, slot)
             w('if (%s) { %s }' % (' && '.join(conds) or 'true', upd), ind + 1)
             w('}', ind)
 
@@ -431,8 +502,14 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
         dpr = Printer(dict((plan.fvars[d], names[d]) for d in range(nd)), cb, mode, lazy=True)
         w('  acc[a * %d] = %s;' % (nthreads, dpr.expr(f.default)))
         w('}')
-        # pre-image of the cell along every owned reduction variable
+        # pre-image of the cell along every owned reduction variable; the
+        # other reduction variables are walked over their whole range
         for j, rv in enumerate(plan.rvars):
+            if rv not in plan.owned:
+                w('const int lo%d = %s, hi%d = %s;' % (
+                    j, base.expr(f.reductionDomain[j].lowerBound), j,
+                    base.expr(f.reductionDomain[j].upperBound)))
+                continue
             d = plan.owned[rv]
             arg = plan.cases[0].red.accumulate_ref.arguments[d]
             rlo = base.expr(f.reductionDomain[j].lowerBound)
@@ -499,7 +576,11 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
         grid = '((%s + %d) / %d)' % (ncell, nthreads - 1, nthreads)
         lds = nthreads * plan.nacc * f.typ.nbytes()
         info = KernelInfo(kname, 'reduction', buffers, grid, nthreads, lds, plan, reads, [f])
-        return wrap_kernel(kname, params, buffers, [f], nthreads, lds, L), info
+        return [(wrap_kernel(kname, params, buffers, [f], nthreads, lds, L), info)]
+
+    if plan.atomic:
+        return _emit_atomic(pipe, plan, kname, mode, params, reads, buffers, exts, lbs, base,
+                            red_body, L)
 
     # serial fallback: one thread, the reference loop order
     w('if (threadIdx.x != 0) return;')
@@ -532,4 +613,77 @@ def emit_reduction_kernel(pipe, plan, kname, mode):
         ind -= 1
         w('}', ind)
     info = KernelInfo(kname, 'reduction', buffers, '1', 64, 0, plan, reads, [f])
-    return wrap_kernel(kname, params, buffers, [f], 64, 0, L), info
+    return [(wrap_kernel(kname, params, buffers, [f], 64, 0, L), info)]
+
+
+_ATOMIC = {Op.Sum: 'atomicAdd', Op.Min: 'atomicMin', Op.Max: 'atomicMax'}
+
+
+def _emit_atomic(pipe, plan, kname, mode, params, reads, buffers, exts, lbs, base, red_body, L):
+    """Two kernels: ``<k>_init`` writes the default value at every target
+    point, ``<k>`` runs one thread per reduction point (the domain
+    flattened in its own order) and applies its value with a device
+    atomic (a compare-and-swap loop for products)."""
+    from .hip_emit import KernelInfo, wrap_kernel
+    f = plan.func
+    T = c_type(f.typ)
+    nd = len(plan.fvars)
+    nthreads = 256
+    # init: one thread per target point
+    I = []
+    wi = lambda t, ind=1: I.append('  ' * ind + t)
+    npts = ' * '.join('(long long)%s' % e for e in exts) or '1'
+    wi('long long p = (long long)pmx_tile * %d + threadIdx.x;' % nthreads)
+    wi('if (p >= %s) return;' % npts)
+    wi('const long long lin = p;')
+    for d in reversed(range(nd)):
+        wi('const int f%d = (int)(p %% %s) + (%s); p /= %s;' % (d, exts[d], lbs[d], exts[d]))
+
+    def cb(ref, pr, pro):
+        g = ref.objectRef
+        lin = None
+        for d, arg in enumerate(ref.arguments):
+            q = pr.expr(arg, pro)
+            t = '(%s - (%s))' % (q, base.expr(g.domain[d].lowerBound))
+            lin = t if lin is None else '(%s * %s + %s)' % (lin, _extent_c(g, d, base), t)
+        return 'G_%s[%s]' % (g.name, lin or '0')
+    dpr = Printer(dict((plan.fvars[d], 'f%d' % d) for d in range(nd)), cb, mode, lazy=True)
+    pro = []
+    dv = dpr.expr(f.default, pro)
+    for l in pro:
+        wi(l)
+    wi('G_%s[lin] = (%s)(%s);' % (f.name, T, dv))
+    init_name = kname + '_init'
+    init_info = KernelInfo(init_name, 'reduction', [f], '((%s + %d) / %d)' % (npts, nthreads - 1, nthreads),
+                           nthreads, 0, plan, [], [f])
+    out = [(wrap_kernel(init_name, params, [f], [f], nthreads, 0, I), init_info)]
+    # accumulation: one thread per reduction point
+    del L[:]
+    w = lambda t, ind=1: L.append('  ' * ind + t)
+    rext = ['((%s) - (%s) + 1)' % (base.expr(iv.upperBound), base.expr(iv.lowerBound))
+            for iv in f.reductionDomain]
+    nred = ' * '.join('(long long)%s' % e for e in rext) or '1'
+    w('long long p = (long long)pmx_tile * %d + threadIdx.x;' % nthreads)
+    w('if (p >= %s) return;' % nred)
+    for j in reversed(range(len(plan.rvars))):
+        w('const int r%d = (int)(p %% %s) + (%s); p /= %s;' % (
+            j, rext[j], base.expr(f.reductionDomain[j].lowerBound), rext[j]))
+    tl = None
+    for d in range(nd):
+        t = '(long long)(q%d - (%s))' % (d, lbs[d])
+        tl = t if tl is None else '(%s * %s + %s)' % (tl, exts[d], t)
+    slot = 'G_%s[%s]' % (f.name, tl or '0')
+
+    def update(op, val):
+        v = '(%s)(%s)' % (T, val)
+        if op in _ATOMIC:
+            return '%s(&%s, %s);' % (_ATOMIC[op], slot, v)
+        # product: compare-and-swap until no other thread intervened
+        return ('{ %s* pa = &%s; const %s m = %s; %s old = *pa, seen; '
+                'do { seen = old; old = atomicCAS(pa, seen, (%s)(seen * m)); } while (old != seen); }'
+                % (T, slot, T, v, T, T))
+    red_body(1, lambda: None, update)
+    info = KernelInfo(kname, 'reduction', buffers, '((%s + %d) / %d)' % (nred, nthreads - 1, nthreads),
+                      nthreads, 0, plan, reads, [f])
+    out.append((wrap_kernel(kname, params, buffers, [f], nthreads, 0, L), info))
+    return out

@@ -162,6 +162,14 @@ class BuiltPipeline(object):
     def batch_destroy(self, handle):
         check(self.lib['pipeline_%s_batch_destroy' % self.name](handle), 'batch_destroy')
 
+    def status(self):
+        """pipeline_<name>_status(): 0, or the negative code of a tile that
+        exceeded its planned LDS box since the last call (the word is then
+        cleared).  Synchronises the null stream."""
+        fn = self.lib['pipeline_%s_status' % self.name]
+        fn.restype = ctypes.c_int
+        return fn()
+
     def run_dev(self, values, stream=0):
         """Device call: values[name] are device pointers (ints) or objects
         with ``data_ptr()``; ``stream`` is a hipStream_t as int."""
@@ -196,26 +204,111 @@ def load(so_path, pipe):
     return BuiltPipeline(pipe, None, so_path)
 
 
-def build(pipe, verbose=False):
-    """Generate, compile (cached) and load a pipeline."""
-    code = pipe.generate_code()
-    key = hashlib.sha256((code.hip_src + '\0' + code.host_src + '\0' +
+def split_source(text):
+    """(hip_src, host_src) of a generated pipeline's source text, the
+    ``str()`` of ``Pipeline.generate_code()`` (kernels, then host stub)."""
+    from .hip_emit import HIP_MARK, HOST_MARK
+    if not text.startswith(HIP_MARK) or HOST_MARK not in text:
+        raise ValueError('not the source of a generated pipeline (missing the kernel / host stub markers)')
+    hip, host = text[len(HIP_MARK):].split(HOST_MARK, 1)
+    return hip, host
+
+
+def build_sources(name, hip_src, host_src):
+    """Compile (cached by source hash) the kernels with hiprtc and the host
+    stub with g++; returns the path of ``lib<name>.so``."""
+    key = hashlib.sha256((hip_src + '\0' + host_src + '\0' +
                           ' '.join(HIPRTC_OPTS) + ARCH).encode()).hexdigest()[:20]
-    d = os.path.join(CACHE, '%s_%s' % (pipe.name, key))
-    so = os.path.join(d, 'lib%s.so' % pipe.name)
+    d = os.path.join(CACHE, '%s_%s' % (name, key))
+    so = os.path.join(d, 'lib%s.so' % name)
     with _lock:
         if not os.path.exists(so):
             os.makedirs(d, exist_ok=True)
             hip = os.path.join(d, 'kernels.hip')
             co = os.path.join(d, 'kernels.co')
             with open(hip, 'w') as f:
-                f.write(code.hip_src)
+                f.write(hip_src)
             if not os.path.exists(co):
-                compile_hip(code.hip_src, co)
+                compile_hip(hip_src, co)
             cpp = os.path.join(d, 'stub.cpp')
             with open(cpp, 'w') as f:
-                f.write(code.host_src.replace('@CO_PATH@', co))
+                f.write(host_src.replace('@CO_PATH@', co))
             tmp = so + '.tmp%d' % os.getpid()
             host_compile(cpp, tmp)
             os.replace(tmp, so)
+    return so
+
+
+def build(pipe, verbose=False):
+    """Generate, compile (cached) and load a pipeline."""
+    code = pipe.generate_code()
+    so = build_sources(pipe.name, code.hip_src, code.host_src)
     return BuiltPipeline(pipe, code, so)
+
+
+# ---------------------------------------------------------------------------
+# the C++/OpenMP backend (cpu_emit): the reference's host code generator
+# ---------------------------------------------------------------------------
+
+class CpuPipeline(object):
+    """A pipeline compiled by the C++/OpenMP backend: ``run_host`` calls
+    ``pipeline_<name>_host`` on host arrays (the reference's entry point,
+    returning a status)."""
+
+    def __init__(self, pipe, src, so_path):
+        self.pipe = pipe
+        self.src = src
+        self.so_path = so_path
+        self.lib = ctypes.CDLL(so_path)
+        self.name = pipe.name
+        self.params = pipe.sorted_parameters()
+        self.inputs = pipe.sorted_inputs()
+        self.outputs = pipe.sorted_outputs()
+        self.fn = self.lib['pipeline_%s_host' % self.name]
+        self.fn.restype = ctypes.c_int
+        self.err = self.lib['pipeline_%s_error' % self.name]
+        self.err.restype = ctypes.c_char_p
+
+    def run_host(self, values):
+        from .utils import convert_to_ctype
+        args = [convert_to_ctype(p.typ.c_type_name(), values[p.name]) for p in self.params]
+        keep = []
+        for f in self.inputs + self.outputs:
+            a = values[f.name]
+            assert isinstance(a, np.ndarray) and a.flags['C_CONTIGUOUS'], f.name
+            assert a.dtype == f.typ.np_dtype(), (f.name, a.dtype)
+            keep.append(a)
+            args.append(ctypes.c_void_p(a.ctypes.data))
+        rc = self.fn(*args)
+        if rc:
+            raise RuntimeError('pipeline_%s (cpu) failed (%d): %s' % (self.name, rc,
+                                                                     self.err().decode(errors='replace')))
+
+
+def build_cpu(pipe, flags=None):
+    """Emit the pipeline as C++/OpenMP (cpu_emit), compile it with g++
+    (cached by source hash and flags; ``-march=native`` code is keyed by the
+    host CPU model too) and load it."""
+    from .cpu_emit import emit_cpu, CPU_FLAGS
+    flags = list(CPU_FLAGS if flags is None else flags)
+    src = emit_cpu(pipe)
+    salt = ''
+    if '-march=native' in flags:
+        try:
+            with open('/proc/cpuinfo') as f:
+                salt = [l for l in f if l.startswith('model name')][0]
+        except (OSError, IndexError):
+            salt = os.uname().nodename
+    key = hashlib.sha256((src + '\0' + ' '.join(flags) + salt).encode()).hexdigest()[:20]
+    d = os.path.join(CACHE, 'cpu_%s_%s' % (pipe.name, key))
+    so = os.path.join(d, 'libcpu_%s.so' % pipe.name)
+    with _lock:
+        if not os.path.exists(so):
+            os.makedirs(d, exist_ok=True)
+            cpp = os.path.join(d, 'pipeline.cpp')
+            with open(cpp, 'w') as f:
+                f.write(src)
+            tmp = so + '.tmp%d' % os.getpid()
+            subprocess.check_call(['g++'] + flags + [cpp, '-o', tmp])
+            os.replace(tmp, so)
+    return CpuPipeline(pipe, src, so)

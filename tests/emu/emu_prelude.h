@@ -30,4 +30,22 @@ static inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __A
 static inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
   return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
 }
+static inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+template <typename T> static inline T pmx_emu_cas(T* p, T cmp, T val) {
+  __atomic_compare_exchange_n(p, &cmp, val, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+  return cmp;
+}
+static inline int atomicCAS(int* p, int cmp, int val) { return pmx_emu_cas(p, cmp, val); }
+static inline unsigned atomicCAS(unsigned* p, unsigned cmp, unsigned val) { return pmx_emu_cas(p, cmp, val); }
+template <typename T> static inline T pmx_emu_minmax(T* p, T v, bool mn) {
+  T old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while ((mn ? v < old : v > old) &&
+         !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+  return old;
+}
+static inline int atomicMin(int* p, int v) { return pmx_emu_minmax(p, v, true); }
+static inline unsigned atomicMin(unsigned* p, unsigned v) { return pmx_emu_minmax(p, v, true); }
+static inline int atomicMax(int* p, int v) { return pmx_emu_minmax(p, v, false); }
+static inline unsigned atomicMax(unsigned* p, unsigned v) { return pmx_emu_minmax(p, v, false); }
 #endif

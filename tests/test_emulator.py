@@ -235,3 +235,53 @@ def test_emulated_batch_handles_under_asan():
                        stderr=subprocess.STDOUT, timeout=900, cwd=os.path.dirname(HERE))
     out = r.stdout.decode(errors='replace')
     assert r.returncode == 0 and 'two handles ok' in out, out[-3000:]
+
+
+@pytest.mark.parametrize("R,C,tiles,threads", [(45, 77, [16, 60], 128), (37, 130, [16, 124], 256),
+                                               (16, 252, [16, 252], 512)])
+def test_emulated_unsharp_forwarded_loads(R, C, tiles, threads):
+    """The benchmark's unsharp shape (one item per thread, 8x1 blocks): the
+    mask takes the image values the blur loaded from registers instead of
+    reading them again; partial tiles and the columns the mask does not
+    cover fall back to loads.  Bit-exact against the oracle, also under
+    AddressSanitizer."""
+    pd = {}
+    out = unsharp_mask_full(pd)
+    pipe = build(out, pd, {'R': R, 'C': C}, 'unsharp', tiles, group_size=5,
+                 gpu_options={'threads': threads, 'block': (8, 1)})
+    assert 'fw_ok' in pipe.generate_code().hip_src
+    img = inputs.smooth_field(np.random.default_rng(R + C), R + 4, C + 4, 3)
+    vals = {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0, 'img': img}
+    o = np.full((3, R, C), -7.0, np.float32)
+    cemit.build([out], 'unsharp_oracle_unsharp_mask_full')(dict(vals, mask=o))
+    for asan in (False, True):
+        if asan and not os.environ.get('PMX_EMU_CHILD'):
+            env = dict(os.environ, PMX_EMU_CHILD='1', ASAN_OPTIONS='detect_leaks=0',
+                       LD_PRELOAD=kernel_emu.asan_runtime())
+            code = ('import sys; sys.path[:0] = [%r, %r]; import test_emulator as t; '
+                    't._forwarded_asan(%d, %d, %r, %d); print("forwarded ok")'
+                    % (HERE, os.path.dirname(HERE), R, C, tiles, threads))
+            r = subprocess.run([sys.executable, '-c', code], env=env, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, timeout=900, cwd=os.path.dirname(HERE))
+            txt = r.stdout.decode(errors='replace')
+            assert r.returncode == 0 and 'forwarded ok' in txt, txt[-3000:]
+            continue
+        if asan:
+            continue
+        g = np.full((3, R, C), -7.0, np.float32)
+        kernel_emu.build(pipe).run_host(dict(vals, mask=g))
+        assert bits_equal(g, o)
+
+
+def _forwarded_asan(R, C, tiles, threads):
+    pd = {}
+    out = unsharp_mask_full(pd)
+    pipe = build(out, pd, {'R': R, 'C': C}, 'unsharp', tiles, group_size=5,
+                 gpu_options={'threads': threads, 'block': (8, 1)})
+    img = inputs.smooth_field(np.random.default_rng(R + C), R + 4, C + 4, 3)
+    vals = {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0, 'img': img}
+    o = np.full((3, R, C), -7.0, np.float32)
+    cemit.build([out], 'unsharp_oracle_unsharp_mask_full')(dict(vals, mask=o))
+    g = np.full((3, R, C), -7.0, np.float32)
+    kernel_emu.build(pipe, asan=True).run_host(dict(vals, mask=g))
+    assert bits_equal(g, o)

@@ -153,3 +153,32 @@ def test_manual_grouping():
         pipe([['grid', 'blurz']])
     with pytest.raises(ValueError, match="cycle"):
         pipe([['blurz', 'blury']])
+
+
+def test_manual_grouping_rejects_unfusible_groups_with_the_reason():
+    """A caller's group in which one function reads another at a
+    data-dependent index along more than 64 points cannot be one kernel:
+    buildPipeline says so, instead of failing later at code generation.
+    Functions of one group asking for different tiles are refused too."""
+    from polymage_amd.compiler import buildPipeline
+    from polymage_amd.constructs import (Parameter, Variable, Interval, Image, Function, Cast,
+                                         Case, Condition, Min, Max, Int, Float)
+    N = Parameter(Int, "N")
+    x = Variable(Int, "x")
+    img = Image(Float, "img", [N])
+    inner = Interval(Int, 1, N - 2)
+    f = Function(([x], [inner]), Float, "f")
+    # (a Case keeps f from being substituted into its data-dependent reader)
+    f.defn = [Case(Condition(x, '>=', 1), img(x - 1) + img(x) + img(x + 1))]
+    g = Function(([x], [inner]), Float, "g")
+    g.defn = [f(Min(Max(Cast(Int, img(x) * 100.0), 1), N - 2)) + 1.0]
+    with pytest.raises(ValueError, match="data-dependent"):
+        buildPipeline([g], param_estimates=[(N, 1000)], grouping=[[f, g]], tile_sizes=[64])
+    h = Function(([x], [Interval(Int, 1, N - 3)]), Float, "h")
+    h.defn = [f(x) + f(x + 1)]
+    p = buildPipeline([h], param_estimates=[(N, 1000)], tile_sizes=[64], group_size=5)
+    assert [[c.func.name for c in grp.comps] for grp in p._groups] == [['f', 'h']]
+    q = buildPipeline([h], param_estimates=[(N, 1000)], tile_sizes=[64], group_size=5,
+                      gpu_options={'group_tiles': {'f': [32], 'h': [64]}})
+    with pytest.raises(ValueError, match="group_tiles"):
+        q.generate_code()
