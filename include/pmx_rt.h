@@ -88,6 +88,22 @@ int pmx_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int pmx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int pmx_memset_d8(void* dst, int value, size_t bytes, void* stream);
 
+/* ---- static scratch (pmx_scratch.hip) ------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): a file-scope device array of
+ * PMX_SCRATCH_BYTES on every GPU that runs the library, held until the
+ * process ends, where the pipeline_<name>_dev_noalloc entry points keep
+ * their status word and intermediates.  pmx_scratch() gives the current
+ * device's address of the array (looked up once per device with
+ * hipGetSymbolAddress, that device current) and its size; pmx_scratch_peek()
+ * gives the address only if it was looked up before (else NULL) and never
+ * touches the device.  All users on one GPU share its array; calls that
+ * overlap on one GPU race. */
+#define PMX_SCRATCH_BYTES (64u << 20)
+int pmx_scratch(void** ptr, size_t* bytes);
+int pmx_scratch_peek(void** ptr);
+
 /* ---- reference pool API kept for callers (video_demo.py:73-83) --------- */
 void pool_init(void);
 void pool_destroy(void);

@@ -172,20 +172,53 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
           % (name, f.name, _size(f, pr), c_type(f.typ), b))
         bufname[f] = b
     w('  void* pmx_err = nullptr; rc = pmx_arena_get_zeroed("%s/err", 64, &pmx_err); if (rc) return rc;' % name)
-    for k in kernels:
-        w('  {')
-        argv = ['&pmx_err']
-        for f in k.buffers:
-            w('    void* a_%s = %s;' % (f.name, bufname[f]))
-            argv.append('&a_%s' % f.name)
-        for p in params:
-            w('    %s a_%s = %s;' % (c_type(p.typ), p.name, p.name))
-            argv.append('&a_%s' % p.name)
-        w('    void* args[] = {%s};' % (', '.join(argv) or 'nullptr'))
-        w('    long long grid = %s;' % k.grid_c)
-        w('    if (grid > 0) { rc = pmx_launch(pmx_fn[dev][%d], (unsigned)grid, 1, 1, %d, 1, 1, 0, stream, args); if (rc) return rc; }'
-          % (kernels.index(k), k.threads))
-        w('  }')
+    _launch_lines(w, kernels, params, bufname)
+    w('  return 0;')
+    w('}')
+
+    # the same launches with every buffer the library provides (status
+    # word, const tables, intermediates) carved from the runtime's static
+    # device scratch: no device allocation after the first call on a device
+    w('extern "C" int pipeline_%s_dev_noalloc(%s) {' % (name, ', '.join(dsig)))
+    w('  int dev = 0; if (hipGetDevice(&dev) != hipSuccess) { pmx_set_error("hipGetDevice"); return -1; }')
+    w('  int rc = pmx_load(dev); if (rc) return rc;')
+    for l in _validation_lines(pipe, pr):
+        w('  ' + l)
+    for f in all_funcs:
+        w('  if (%s > (long long)INT_MAX) { pmx_set_error("%s: array too large for 32-bit indexing"); return -75; }'
+          % (_size(f, pr), f.name))
+    w('  void* pmx_scr = nullptr; size_t pmx_cap = 0; rc = pmx_scratch(&pmx_scr, &pmx_cap); if (rc) return rc;')
+    w('  size_t pmx_off = 256;  // [0, 64): the status word of the allocation-free calls')
+    nb = {}
+    for f in inputs:
+        nb[f] = '(void*)%s_dev' % f.name
+    for f in outputs:
+        nb[f] = '%s_dev' % f.name
+    offs = []
+    for cp in host_consts:
+        f = cp.func
+        w('  const %s* h_%s = nullptr; size_t n_%s = 0;' % (c_type(f.typ), f.name, f.name))
+        w('  rc = %s_host(%s%s&h_%s, &n_%s, nullptr); if (rc) return rc;' % (
+            cp.upload_fn, ', '.join(p.name for p in cp.params), ', ' if cp.params else '', f.name, f.name))
+        w('  const size_t o_%s = pmx_off; pmx_off += (n_%s + 255) & ~(size_t)255;' % (f.name, f.name))
+        offs.append(f)
+    for f in all_funcs:
+        if f in nb or f in offs:
+            continue
+        w('  const size_t o_%s = pmx_off; pmx_off += ((size_t)(%s) * sizeof(%s) + 255) & ~(size_t)255;'
+          % (f.name, _size(f, pr), c_type(f.typ)))
+        offs.append(f)
+    w('  if (pmx_off > pmx_cap) { char m[256]; snprintf(m, sizeof m, "%s: these sizes need %%zu bytes of '
+      'static scratch, which holds %%zu; use pipeline_%s_dev", pmx_off, pmx_cap); pmx_set_error(m); return -12; }'
+      % (name, name))
+    for f in offs:
+        nb[f] = 'b_%s' % f.name
+        w('  void* b_%s = (void*)((unsigned char*)pmx_scr + o_%s);' % (f.name, f.name))
+    for cp in host_consts:
+        f = cp.func
+        w('  rc = pmx_memcpy_h2d(b_%s, h_%s, n_%s, stream); if (rc) return rc;' % (f.name, f.name, f.name))
+    w('  void* pmx_err = pmx_scr;')
+    _launch_lines(w, kernels, params, nb)
     w('  return 0;')
     w('}')
 
@@ -196,7 +229,13 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
     w('  void* e = nullptr; int rc = pmx_arena_get_zeroed("%s/err", 64, &e); if (rc) return rc;' % name)
     w('  int v = 0; rc = pmx_memcpy_d2h(&v, e, sizeof v, nullptr); if (rc) return rc;')
     w('  rc = pmx_stream_sync(nullptr); if (rc) return rc;')
-    w('  if (v) { int z = 0; pmx_memcpy_h2d(e, &z, sizeof z, nullptr); pmx_stream_sync(nullptr);')
+    w('  if (v) { int z = 0; pmx_memcpy_h2d(e, &z, sizeof z, nullptr); pmx_stream_sync(nullptr); }')
+    w('  // the allocation-free calls keep their status word in the static scratch')
+    w('  void* s = nullptr; pmx_scratch_peek(&s);')
+    w('  if (s) { int u = 0; rc = pmx_memcpy_d2h(&u, s, sizeof u, nullptr); if (rc) return rc;')
+    w('    rc = pmx_stream_sync(nullptr); if (rc) return rc;')
+    w('    if (u) { int z = 0; pmx_memcpy_h2d(s, &z, sizeof z, nullptr); pmx_stream_sync(nullptr); v |= u; } }')
+    w('  if (v) {')
     w('    pmx_set_error("%s: a tile exceeded its planned LDS box; rebuild with estimates closer to these sizes"); return -34; }' % name)
     w('  return 0;')
     w('}')
@@ -223,6 +262,25 @@ def emit_host_stub(pipe, kernels, host_consts, co_symbol='pmx_co', co_path='@CO_
     w('  if (rc) fprintf(stderr, "pipeline_%s: %%s\\n", pmx_last_error());' % name)
     w('}')
     return '\n'.join(L) + '\n'
+
+
+def _launch_lines(w, kernels, params, bufname):
+    """One launch per kernel, in order, on ``stream``; ``pmx_err``, ``dev``
+    and ``rc`` are in scope."""
+    for k in kernels:
+        w('  {')
+        argv = ['&pmx_err']
+        for f in k.buffers:
+            w('    void* a_%s = %s;' % (f.name, bufname[f]))
+            argv.append('&a_%s' % f.name)
+        for p in params:
+            w('    %s a_%s = %s;' % (c_type(p.typ), p.name, p.name))
+            argv.append('&a_%s' % p.name)
+        w('    void* args[] = {%s};' % (', '.join(argv) or 'nullptr'))
+        w('    long long grid = %s;' % k.grid_c)
+        w('    if (grid > 0) { rc = pmx_launch(pmx_fn[dev][%d], (unsigned)grid, 1, 1, %d, 1, 1, 0, stream, args); if (rc) return rc; }'
+          % (kernels.index(k), k.threads))
+        w('  }')
 
 
 def frame_struct_fields(kernel, params):

@@ -162,3 +162,20 @@ extern "C" int pmx_mlp_forward(const float*, const int*, int, const float*, int,
 extern "C" int pmx_mlp_backward(const float*, const int*, int, const float*, const float*, int, float*, float*, void*) { return fail("emulator"); }
 extern "C" int pmx_mlp_adam(float*, const float*, float*, float*, int, float, float, float, float, int, void*) { return fail("emulator"); }
 extern "C" int pmx_mlp_train(float*, const int*, int, const float*, const float*, int, int, float, float, float, float, float*, float*, float*, float*, float*, int, void*) { return fail("emulator"); }
+
+// static scratch of the allocation-free entry points: one zero-initialised
+// host array standing in for the per-device array
+namespace {
+alignas(256) unsigned char g_emu_scratch[PMX_SCRATCH_BYTES];
+bool g_emu_scratch_used = false;
+}  // namespace
+extern "C" int pmx_scratch(void** p, size_t* bytes) {
+  *p = g_emu_scratch;
+  if (bytes) *bytes = sizeof g_emu_scratch;
+  g_emu_scratch_used = true;
+  return 0;
+}
+extern "C" int pmx_scratch_peek(void** p) {
+  *p = g_emu_scratch_used ? (void*)g_emu_scratch : nullptr;
+  return 0;
+}

@@ -43,6 +43,6 @@ def test_pipeline_libraries_export_every_declared_entry_point():
     assert not missing, missing
     # and every generated library has the full generic set
     for wl, lib in libs.items():
-        for suffix in ('', '_host', '_dev', '_status', '_batch_create', '_batch_run',
+        for suffix in ('', '_host', '_dev', '_dev_noalloc', '_status', '_batch_create', '_batch_run',
                        '_batch_run_graph', '_batch_run_frames', '_batch_destroy'):
             assert hasattr(lib, 'pipeline_%s%s' % (wl, suffix)), (wl, suffix)
