@@ -544,13 +544,14 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe, args.steps)
         return
-    # the CPU baseline's OpenMP threads stay on neighbouring cores (set
-    # before the first OpenMP library loads)
+    # the host's cores as this process sees them, read before any OpenMP
+    # runtime starts (a bound initial thread would report one CPU); the
+    # CPU baseline's threads then stay on neighbouring cores
+    cpu = host_cpu()
     os.environ.setdefault('OMP_PROC_BIND', 'close')
     import torch
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
-    cpu = host_cpu()
     copy_gbs = device_copy_gbs(torch, dev)
     ok = True
 

@@ -216,8 +216,10 @@ def schedule(app, which='learned'):
     rec = tuner.tuned_schedule(app, which) if which != 'default' else None
     if rec is None:
         return tuple(DEFAULT_TILES[app]), None, None, 'default'
-    return tuple(rec['tiles']), rec['group_size'], {'threads': rec['threads']}, \
-        '%s: %s' % (which, rec['key'])
+    gopts = {'threads': rec['threads']}
+    if rec.get('lanes'):
+        gopts['lanes'] = 1
+    return tuple(rec['tiles']), rec['group_size'], gopts, '%s: %s' % (which, rec['key'])
 
 
 def tuned(app, which='learned', **kw):

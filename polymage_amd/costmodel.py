@@ -152,7 +152,7 @@ def features(pipe):
             slots = 256.0 * max(1, min(occ[-1], 2048 // threads))
             rounds = math.ceil(p.total_tiles / slots)
             tail.append(1.0 - p.total_tiles / (rounds * slots))
-            barriers = max(barriers, p.nphases - 1)
+            barriers = max(barriers, p.nphases - 1 - (getattr(p, 'lane_pair', None) is not None))
             for s in p.stages:
                 if s.liveout and s.func not in outs:
                     n = 1
@@ -210,7 +210,8 @@ def features(pipe):
 
 
 def vector(feats):
-    return np.array([float(feats[k]) for k in FEATURES], dtype=np.float32)
+    # (records written before a feature existed read it as 0)
+    return np.array([float(feats.get(k, 0.0)) for k in FEATURES], dtype=np.float32)
 
 
 # ---------------------------------------------------------------------------

@@ -158,4 +158,12 @@ __device__ __forceinline__ double pmx_cos(int v) { return cos((double)v); }
 __device__ __forceinline__ int pmx_imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int pmx_imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int pmx_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// value of v in lane src of the wavefront (lane fusion): 32- and 64-bit
+// types move as they are, narrower integers through int
+template <typename T> __device__ __forceinline__ T pmx_shfl(T v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ unsigned char pmx_shfl(unsigned char v, int src) { return (unsigned char)__shfl((int)v, src, 64); }
+__device__ __forceinline__ signed char pmx_shfl(signed char v, int src) { return (signed char)__shfl((int)v, src, 64); }
+__device__ __forceinline__ char pmx_shfl(char v, int src) { return (char)__shfl((int)v, src, 64); }
+__device__ __forceinline__ unsigned short pmx_shfl(unsigned short v, int src) { return (unsigned short)__shfl((int)v, src, 64); }
+__device__ __forceinline__ short pmx_shfl(short v, int src) { return (short)__shfl((int)v, src, 64); }
 '''

@@ -57,6 +57,7 @@ class StageInfo(object):
         self.lds_offset = None
         self.lds_class = None
         self.needs_lds = False
+        self.lane = False               # kept in registers, read across lanes
 
     def reset(self):
         """Forget the analysis results (parts and live-out flag stay)."""
@@ -70,6 +71,7 @@ class StageInfo(object):
         self.lds_offset = None
         self.lds_class = None
         self.needs_lds = False
+        self.lane = False
 
     @property
     def name(self):
@@ -147,6 +149,7 @@ class GroupPlan(object):
                 self._build(stage=False)
                 return
         self._blocks()
+        self._lane_fuse()
         self._storage()
 
     def _analyse(self):
@@ -572,7 +575,20 @@ class GroupPlan(object):
                 for i in range(s.nd):
                     lo[i] = ir.min(lo[i], own[i][0])
                     hi[i] = ir.max(hi[i], own[i][1])
+            # the references of one consumer part share its sub-region, so
+            # their requests are united first and guarded once; requests
+            # that shift the same sub-region bound by constants fold into
+            # that bound plus the extreme shift (a 3x3 stencil's nine taps
+            # give one min and one max per dimension, not nine)
+            groups = []
+            gidx = {}
             for (c, part, r) in s.consumers:
+                k = (id(c), id(part))
+                if k not in gidx:
+                    gidx[k] = len(groups)
+                    groups.append((c, part, []))
+                groups[gidx[k]][2].append(r)
+            for (c, part, refs) in groups:
                 # sub-region of the consumer that this part covers
                 sub = []
                 for j in range(c.nd):
@@ -584,21 +600,43 @@ class GroupPlan(object):
                         chi = ir.min(chi, ir.expr(bhi))
                     sub.append((clo, chi))
                 valid = [(l, h) for (l, h) in sub]
-                for i, arg in enumerate(r.arguments):
-                    acc = classify_access(arg, c.func.variables)
-                    if acc.kind == 'var':
-                        l, h = sub[acc.dim]
-                        al = ir.access(arg, acc.var, l)
-                        ah = ir.access(arg, acc.var, h)
-                        if not acc.increasing:
-                            al, ah = ah, al
-                    elif acc.kind == 'const':
-                        al = ah = ir.expr(arg)
-                    else:
-                        al, ah = dom_ids(s, i)
-                    lo[i] = ir.sel_valid(valid, ir.min(lo[i], al), lo[i])
-                    hi[i] = ir.sel_valid(valid, ir.max(hi[i], ah), hi[i])
-                    self._requests.append((r, i, s, valid, al, ah))
+                want_lo = [None] * s.nd
+                want_hi = [None] * s.nd
+                shifts = {}
+                for r in refs:
+                    for i, arg in enumerate(r.arguments):
+                        acc = classify_access(arg, c.func.variables)
+                        sa = shift_access(arg, c.func.variables)
+                        if acc.kind == 'var':
+                            l, h = sub[acc.dim]
+                            al = ir.access(arg, acc.var, l)
+                            ah = ir.access(arg, acc.var, h)
+                            if not acc.increasing:
+                                al, ah = ah, al
+                        elif acc.kind == 'const':
+                            al = ah = ir.expr(arg)
+                        else:
+                            al, ah = dom_ids(s, i)
+                        self._requests.append((r, i, s, valid, al, ah))
+                        if acc.kind == 'var' and sa is not None and sa[0] != 'const' \
+                                and isinstance(sa[1], int):
+                            key = (i, sa[0])
+                            mn, mx = shifts.get(key, (sa[1], sa[1]))
+                            shifts[key] = (min(mn, sa[1]), max(mx, sa[1]))
+                            continue
+                        want_lo[i] = al if want_lo[i] is None else ir.min(want_lo[i], al)
+                        want_hi[i] = ah if want_hi[i] is None else ir.max(want_hi[i], ah)
+                for (i, j), (mn, mx) in sorted(shifts.items()):
+                    l, h = sub[j]
+                    al = ir.add(l, ir.const(mn)) if mn else l
+                    ah = ir.add(h, ir.const(mx)) if mx else h
+                    want_lo[i] = al if want_lo[i] is None else ir.min(want_lo[i], al)
+                    want_hi[i] = ah if want_hi[i] is None else ir.max(want_hi[i], ah)
+                for i in range(s.nd):
+                    if want_lo[i] is None:
+                        continue
+                    lo[i] = ir.sel_valid(valid, ir.min(lo[i], want_lo[i]), lo[i])
+                    hi[i] = ir.sel_valid(valid, ir.max(hi[i], want_hi[i]), hi[i])
             reg = []
             for i in range(s.nd):
                 dlo, dhi = dom_ids(s, i)
@@ -837,6 +875,73 @@ class GroupPlan(object):
                     groups.append([s])
             for g in groups:
                 self.sweeps.append((ph, g))
+
+    # widest column reach a lane-fused sweep reads across lanes
+    LANE_REACH = 3
+
+    def _lane_fuse(self):
+        """Lane fusion (gpu option ``lanes``): a sweep whose stages are read
+        only by the next phase's single stored stage, at small offsets
+        (rows: any; innermost dimension: at most LANE_REACH), is not stored
+        in LDS.  Each thread of the consumer's sweep computes those stages
+        in registers for its column, on the rows its block needs; values of
+        neighbouring columns come from neighbouring lanes of the wavefront
+        (a lane shuffle).  A wavefront covers 64 consecutive columns, of
+        which the 2*reach at its edges only feed their neighbours.  The
+        barrier between the two phases and the LDS traffic go away.
+        ``self.lane_pair`` = (producer sweep, consumer sweep, row reach below,
+        row reach above, column reach) or None."""
+        self.lane_pair = None
+        if not self.opts.get('lanes'):
+            return
+        for li in range(len(self.sweeps) - 1):
+            ph, prod = self.sweeps[li]
+            ph2, cons = self.sweeps[li + 1]
+            if ph2 != ph + 1 or len(cons) != 1:
+                continue
+            if any(q[0] == ph for q in self.sweeps if q is not self.sweeps[li]) or \
+                    any(q[0] == ph2 for q in self.sweeps if q is not self.sweeps[li + 1]):
+                continue
+            c = cons[0]
+            U, V = c.block
+            if V != 1 or c.bdims[1] is None:
+                continue
+            rd, cd = c.bdims
+            if any(m.liveout or m.nd != c.nd or m.bdims != c.bdims for m in prod):
+                continue
+            mset = set(prod)
+            lo = hi = reach = 0
+            ok = True
+            closure = set([c])
+            for (t, r) in self.closure_refs(c):
+                if self.in_group(r.objectRef) and self.stage_of(r.objectRef).inline:
+                    closure.add(self.stage_of(r.objectRef))
+            for m in prod:
+                for (cs, part, r) in m.consumers:
+                    if cs not in closure:
+                        ok = False
+            for (t, r) in self.closure_refs(c):
+                if not (self.in_group(r.objectRef) and self.stage_of(r.objectRef) in mset):
+                    continue
+                for j, arg in enumerate(r.arguments):
+                    sa = shift_access(arg, t.func.variables)
+                    if sa is None or sa[0] != j or not isinstance(sa[1], int):
+                        ok = False
+                        break
+                    if j == cd:
+                        reach = max(reach, abs(sa[1]))
+                    elif j == rd:
+                        lo = min(lo, sa[1])
+                        hi = max(hi, sa[1])
+                    elif sa[1] != 0:
+                        ok = False
+            if not ok or reach > self.LANE_REACH:
+                continue
+            for m in prod:
+                m.needs_lds = False
+                m.lane = True
+            self.lane_pair = (li, li + 1, -lo, hi, reach)
+            return
 
     def lds_ext(self, s):
         """LDS layout extents of a stage: the static box, with the innermost

@@ -259,6 +259,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             # or, for a later sweep, replaced by the register holding them
             self.record = None
             self.forward = None
+            self.lane_keys = None       # lane-fused reads this item makes
 
         def _base(self, key, make):
             if key not in self.names:
@@ -304,6 +305,20 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         def ref(self, f, sh, pr):
             kinds = tuple(x[0] for x in sh)
             parts = []
+            if plan.in_group(f) and plan.stage_of(f).lane:
+                # a lane-fused producer: this thread's registers (its column)
+                # or a neighbouring lane's (other columns)
+                p = plan.stage_of(f)
+                lp = plan.lane_pair
+                rd_, cd_ = p.bdims
+                row = self.d[rd_] + sh[rd_][1] + lp[2]
+                dc = sh[cd_][1]
+                nm = 'lv_%s_%d' % (p.name, row)
+                if dc:
+                    nm = 'lx_%s_%d_%s%d' % (p.name, row, 'm' if dc < 0 else 'p', abs(dc))
+                if self.lane_keys is not None:
+                    self.lane_keys.add((p.name, row, dc))
+                return nm
             if plan.in_group(f):
                 p = plan.stage_of(f)
                 ext = plan.lds_ext(p)
@@ -519,7 +534,14 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     fw_need = set()     # keys the later sweep's fast path reads
     fw_vars = {}        # key -> register name
     phase_code = {}     # phase -> [(comment, nitems, lines)]
+    lp = getattr(plan, 'lane_pair', None)
     for si, (ph, members) in enumerate(plan.sweeps):
+        if lp is not None and si == lp[0]:
+            continue            # computed inside the consumer's sweep
+        if lp is not None and si == lp[1]:
+            phase_code.setdefault(ph, []).append(
+                _lane_sweep(plan, si, sweep_box[si], Item, emit_point, emit_store, R, threads))
+            continue
         SL = []
         w3 = lambda t, ind=3: SL.append('  ' * ind + t)
         s0 = members[0]
@@ -694,12 +716,14 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
     # one loop per phase over the concatenated item spaces of its sweeps, so
     # several small sweeps share the workgroup's lanes instead of each
     # leaving most of them idle in its last iteration
+    first = True
     for ph in range(plan.nphases):
         sws = phase_code.get(ph, [])
-        if ph > 0:
-            w('__syncthreads();')
         if not sws:
             continue
+        if not first:
+            w('__syncthreads();')
+        first = False
         total_items = sum(n for (_, n, _) in sws)
         w('{ // phase %d: %s' % (ph, '; '.join(c for (c, _, _) in sws)))
         w('for (int pmx_it = pmx_tx; pmx_it < %d; pmx_it += %d) {' % (total_items, threads), 2)
@@ -730,6 +754,161 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
 
 
 
+def _lane_sweep(plan, si, box, Item, emit_point, emit_store, R, threads):
+    """Code of a consumer sweep with its producer sweep lane-fused
+    (group_plan._lane_fuse): (comment, items, lines).
+
+    Items run over the consumer's region: outer dimensions, blocks of U
+    rows, then columns in wavefront-wide groups of 64 lanes whose
+    consecutive columns overlap by 2*reach (a wavefront advances by
+    64 - 2*reach columns), so every item of a wavefront shares its outer
+    index and row block.  Each item computes the producer stages for its
+    column on rows -below .. U-1+above into registers, exchanges them with
+    the lanes ``reach`` to each side (all lanes take part: no branch
+    encloses the exchange), and lanes reach .. 63-reach compute the
+    consumer's U points."""
+    lp = plan.lane_pair
+    ph, members = plan.sweeps[si]
+    c = members[0]
+    prod = plan.sweeps[lp[0]][1]
+    below, above, B = lp[2], lp[3], lp[4]
+    U = c.block[0]
+    rd, cd = c.bdims
+    nd = c.nd
+    ext = c.static_ext
+    Wv = 64 - 2 * B
+    cnt = list(ext)
+    cnt[rd] = -(-ext[rd] // U)
+    cnt[cd] = -(-ext[cd] // Wv) * 64
+    nitems = _prod(cnt)
+    SL = []
+    w3 = lambda t, ind=3: SL.append('  ' * ind + t)
+    rem = 'it'
+    pb = []
+    for i in range(nd):
+        st = _prod(cnt[i + 1:])
+        if cnt[i] == 1:
+            w3('const int j%d = 0;' % i)
+        elif i < nd - 1:
+            mg = _magic24(cnt[i] * st, st)
+            if mg:
+                w3('const int j%d = (int)(__umul24((unsigned)%s, %du) >> %d); '
+                   'const int k%d = %s - __mul24(j%d, %d);' % (i, rem, mg[0], mg[1], i, rem, i, st))
+            else:
+                w3('const int j%d = %s / %d; const int k%d = %s - j%d * %d;' % (i, rem, st, i, rem, i, st))
+            rem = 'k%d' % i
+        else:
+            w3('const int j%d = %s;' % (i, rem))
+        if i == cd:
+            w3('const int pmx_lane = j%d & 63;' % i)
+            w3('const int b%d = %s - %d + (j%d >> 6) * %d + pmx_lane;' % (i, box[i][0], B, i, Wv))
+        elif i == rd:
+            w3('const int b%d = %s + j%d * %d;' % (i, box[i][0], i, U))
+        else:
+            w3('const int b%d = %s + j%d;' % (i, box[i][0], i))
+        pb.append('b%d' % i)
+    # whole wavefronts share the outer index and the row block
+    og = ' || '.join('b%d > %s' % (i, box[i][1]) for i in range(nd) if i != cd)
+    if og:
+        w3('if (%s) continue;' % og)
+
+    def dvec(u):
+        d = [0] * nd
+        d[rd] = u
+        return d
+    rows = list(range(-below, U + above))
+    item = Item(pb, dvec(0))
+    # producer values on the rows of this block, this column
+    for m in prod:
+        T = c_type(m.func.typ)
+        for u in rows:
+            w3('%s lv_%s_%d = (%s)0;' % (T, m.name, u + below, T))
+    fast_c = []
+    for m in prod:
+        lo_r, hi_r = R(m.region[rd][0]), R(m.region[rd][1])
+        lo_c, hi_c = R(m.region[cd][0]), R(m.region[cd][1])
+        fast_c += ['b%d - %d >= %s' % (rd, below, lo_r), 'b%d + %d <= %s' % (rd, U - 1 + above, hi_r),
+                   'b%d >= %s' % (cd, lo_c), 'b%d <= %s' % (cd, hi_c)]
+        for i in range(nd):
+            if i not in (rd, cd):
+                fast_c += ['b%d >= %s' % (i, R(m.region[i][0])), 'b%d <= %s' % (i, R(m.region[i][1]))]
+    body = []
+    for m in prod:
+        for u in rows:
+            item.d = dvec(u)
+            emit_point(body, m, item, 4, 'lv_%s_%d' % (m.name, u + below), 'pmx_lw')
+    slow = []
+    for m in prod:
+        for u in rows:
+            conds = ['b%d + %d >= %s' % (rd, u, R(m.region[rd][0])),
+                     'b%d + %d <= %s' % (rd, u, R(m.region[rd][1])),
+                     'b%d >= %s' % (cd, R(m.region[cd][0])), 'b%d <= %s' % (cd, R(m.region[cd][1]))]
+            for i in range(nd):
+                if i not in (rd, cd):
+                    conds += ['b%d >= %s' % (i, R(m.region[i][0])), 'b%d <= %s' % (i, R(m.region[i][1]))]
+            slow.append('  ' * 4 + 'if (%s) {' % ' && '.join(conds))
+            item.d = dvec(u)
+            emit_point(slow, m, item, 5, 'lv_%s_%d' % (m.name, u + below), 'pmx_lw')
+            slow.append('  ' * 4 + '}')
+    for dcl in item.decls:
+        w3(dcl)
+    w3('{ bool pmx_lw = false; (void)pmx_lw;')
+    w3('if (%s) {' % ' && '.join(fast_c))
+    SL.extend(body)
+    w3('} else {')
+    SL.extend(slow)
+    w3('}')
+    w3('}')
+    # the consumer's points: which neighbouring values they read
+    probe = Item(pb, dvec(0))
+    probe.lane_keys = set()
+    for u in range(U):
+        probe.d = dvec(u)
+        emit_point([], c, probe, 4, 'x', 'y')
+    typ = dict((m.name, c_type(m.func.typ)) for m in prod)
+    for (mn, row, dc) in sorted(probe.lane_keys):
+        if dc:
+            w3('const %s lx_%s_%d_%s%d = pmx_shfl(lv_%s_%d, pmx_lane + (%d));' % (
+                typ[mn], mn, row, 'm' if dc < 0 else 'p', abs(dc), mn, row, dc))
+    citem = Item(pb, dvec(0))
+    citem.names = item.names        # one namespace of base addresses per item
+    pts = []
+    for u in range(U):
+        T = c_type(c.func.typ)
+        pts.append('  ' * 4 + '%s pv_%d = (%s)0; bool pw_%d = false;' % (T, u, T, u))
+    fast = list(pts)
+    for u in range(U):
+        citem.d = dvec(u)
+        emit_point(fast, c, citem, 4, 'pv_%d' % u, 'pw_%d' % u)
+    for u in range(U):
+        citem.d = dvec(u)
+        emit_store(fast, c, citem, 4, 'pv_%d' % u, 'pw_%d' % u)
+    cslow = []
+    for u in range(U):
+        conds = ['b%d + %d <= %s' % (rd, u, box[rd][1]), 'b%d >= %s' % (cd, box[cd][0]),
+                 'b%d <= %s' % (cd, box[cd][1])]
+        cslow.append('  ' * 4 + 'if (%s) {' % ' && '.join(conds))
+        T = c_type(c.func.typ)
+        cslow.append('  ' * 5 + '%s pv_s = (%s)0; bool pw_s = false;' % (T, T))
+        citem.d = dvec(u)
+        emit_point(cslow, c, citem, 5, 'pv_s', 'pw_s')
+        emit_store(cslow, c, citem, 5, 'pv_s', 'pw_s')
+        cslow.append('  ' * 4 + '}')
+    for dcl in citem.decls:
+        w3(dcl)
+    w3('if (pmx_lane >= %d && pmx_lane < %d) {' % (B, 64 - B))
+    w3('if (b%d + %d <= %s && b%d >= %s && b%d <= %s) {' % (rd, U - 1, box[rd][1], cd, box[cd][0],
+                                                          cd, box[cd][1]), 4)
+    SL.extend(fast)
+    w3('} else {', 4)
+    SL.extend(cslow)
+    w3('}', 4)
+    w3('}')
+    comment = 'stage %s, with %s in registers across lanes (phase %d, %dx1 points per thread)' % (
+        c.name, ' + '.join(m.name for m in prod), ph, U)
+    return (comment, nitems, SL)
+
+
 _FW_LINE = re.compile(r'^fw\d+ = ')
 
 
@@ -743,7 +922,7 @@ def _forward_pair(plan, threads):
     mask reads the image at the points the blur loaded; without this the
     second read goes back to L2."""
     import numpy as np
-    if plan.opts.get('no_forward'):
+    if plan.opts.get('no_forward') or getattr(plan, 'lane_pair', None) is not None:
         return None
     by_phase = {}
     for si, (ph, members) in enumerate(plan.sweeps):

@@ -196,25 +196,35 @@ def execute(arg):
 # ---------------------------------------------------------------------------
 
 class Candidate(object):
-    def __init__(self, tiles, group_size, threads=256):
+    def __init__(self, tiles, group_size, threads=256, lanes=False):
         self.tiles = list(tiles)
         self.group_size = group_size
         self.threads = threads
+        self.lanes = bool(lanes)
 
     @property
     def key(self):
-        return 'T%s_g%d_w%d' % ('x'.join(map(str, self.tiles)), self.group_size, self.threads)
+        return 'T%s_g%d_w%d%s' % ('x'.join(map(str, self.tiles)), self.group_size, self.threads,
+                                  '_L' if self.lanes else '')
+
+    @property
+    def gpu_options(self):
+        o = {'threads': self.threads}
+        if self.lanes:
+            o['lanes'] = 1
+        return o
 
     def __repr__(self):
         return 'Candidate(%s)' % self.key
 
 
 class Space(object):
-    """Cartesian product of tile sizes, group sizes and threads."""
+    """Cartesian product of tile sizes, group sizes, threads and (for
+    applications with a lane-fusable group) lane fusion on or off."""
 
-    def __init__(self, tiles, group_sizes, threads=(256,)):
-        self.candidates = [Candidate(t, g, w) for t, g, w in
-                           itertools.product(tiles, group_sizes, threads)]
+    def __init__(self, tiles, group_sizes, threads=(256,), lanes=(False,)):
+        self.candidates = [Candidate(t, g, w, l) for t, g, w, l in
+                           itertools.product(tiles, group_sizes, threads, lanes)]
 
     def __iter__(self):
         return iter(self.candidates)
@@ -271,17 +281,42 @@ def aligned_tiles(app):
     return [(a, b - h) for (a, b) in POW2_TILES if b >= 32 and b - h > h]
 
 
+_LANE_REACH = {}
+
+
+def lane_tiles(app):
+    """Tiles whose innermost size is a whole number of lane-fused wavefront
+    steps (64 - 2 * column reach: unsharp 60, Harris 62), or [] when the
+    application has no lane-fusable group."""
+    if app not in _LANE_REACH:
+        from .apps import workloads
+        from .hip_emit import plan_pipeline
+        from .group_plan import GroupPlan
+        w = workloads.BUILDERS[app](with_data=False, gpu_options={'lanes': 1})
+        reach = None
+        for p in plan_pipeline(w.pipe):
+            if isinstance(p, GroupPlan) and getattr(p, 'lane_pair', None) is not None:
+                reach = p.lane_pair[4]
+        _LANE_REACH[app] = reach
+    r = _LANE_REACH[app]
+    if r is None:
+        return []
+    step = 64 - 2 * r
+    return [(a, k * step) for a in (8, 16, 32, 64, 128) for k in (1, 2, 4, 8)]
+
+
 def default_space(app):
     """Tile sizes {8..512}^2 (powers of two, as the benchmark configuration
     states) and the same with the innermost size less the application's
     halo (``aligned_tiles``), times the application's group sizes times
-    workgroup sizes.  Schedules that do not fit (LDS) fail at planning and
-    are skipped."""
+    workgroup sizes, times lane fusion on/off where the application's main
+    group has a lane-fusable pair of sweeps (unsharp, Harris).  Schedules
+    that do not fit (LDS) fail at planning and are skipped."""
     tiles = POW2_TILES + aligned_tiles(app)
     if app == 'unsharp':
-        return Space(tiles, [1, 5], (128, 256, 512))
+        return Space(tiles + lane_tiles(app), [1, 5], (128, 256, 512), lanes=(False, True))
     if app == 'harris':
-        return Space(tiles, [1, 5, 11], (256, 512))
+        return Space(tiles + lane_tiles(app), [1, 5, 11], (256, 512), lanes=(False, True))
     if app == 'bilateral':
         return Space(tiles, [1, 5], (128, 256))
     if app == 'laplacian':
@@ -298,7 +333,7 @@ def default_space(app):
 def build_candidate(workload_fn, cand, with_data=False):
     """Build the pipeline of ``workload_fn`` for one candidate."""
     return workload_fn(tiles=tuple(cand.tiles), group_size=cand.group_size,
-                       gpu_options={'threads': cand.threads}, with_data=with_data)
+                       gpu_options=cand.gpu_options, with_data=with_data)
 
 
 def _worker_init(cache):
@@ -493,10 +528,12 @@ TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuned.jso
 
 
 def parse_key(key):
-    """'T32x256_g5_w256' -> {'tiles': (32, 256), 'group_size': 5, 'threads': 256}."""
-    t, g, w = key.split('_')
+    """'T32x256_g5_w256' -> {'tiles': (32, 256), 'group_size': 5, 'threads': 256,
+    'lanes': False}; a trailing '_L' marks lane fusion."""
+    parts = key.split('_')
+    t, g, w = parts[:3]
     return {'tiles': tuple(int(x) for x in t[1:].split('x')), 'group_size': int(g[1:]),
-            'threads': int(w[1:]), 'key': key}
+            'threads': int(w[1:]), 'lanes': parts[3:] == ['L'], 'key': key}
 
 
 def tuned_schedule(app, which='learned', path=None):

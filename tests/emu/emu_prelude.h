@@ -18,6 +18,23 @@ extern pmx_dim3 gridDim;
 #define __shared__ static
 #define __align__(n) __attribute__((aligned(n)))
 void __syncthreads();
+void pmx_emu_wave_sync();
+extern unsigned long long* pmx_emu_xchg;
+#include <cstring>
+// HIP's __shfl on a wavefront of 64 threads (every lane takes part)
+template <typename T> static inline T __shfl(T v, int src, int width = 64) {
+  static_assert(sizeof(T) <= 8, "__shfl of a wider type");
+  (void)width;
+  unsigned long long w = 0;
+  memcpy(&w, &v, sizeof(T));
+  pmx_emu_xchg[threadIdx.x] = w;
+  pmx_emu_wave_sync();
+  const unsigned long long r = pmx_emu_xchg[(threadIdx.x & ~63u) + ((unsigned)src & 63u)];
+  pmx_emu_wave_sync();
+  T out;
+  memcpy(&out, &r, sizeof(T));
+  return out;
+}
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 #define __builtin_amdgcn_readfirstlane(v) (v)
 #define PMX_GLOBAL

@@ -32,6 +32,12 @@ static int fail(const char* m) { pmx_set_error(m); return -1; }
 
 void __syncthreads() { pthread_barrier_wait(g_bar); }
 
+// lane exchange of a wavefront (__shfl): 64 threads share an exchange row
+// and meet at their own barrier
+unsigned long long* pmx_emu_xchg = nullptr;
+static std::vector<pthread_barrier_t>* g_wbar = nullptr;
+void pmx_emu_wave_sync() { pthread_barrier_wait(&(*g_wbar)[threadIdx.x / 64]); }
+
 // ---- kernel registry ------------------------------------------------------
 typedef void (*pmx_tramp)(void**);
 static std::map<std::string, pmx_tramp>& registry() {
@@ -114,6 +120,13 @@ extern "C" int pmx_launch(void* function, unsigned gx, unsigned gy, unsigned gz,
   pthread_barrier_t bar;
   pthread_barrier_init(&bar, nullptr, bx);
   g_bar = &bar;
+  // one barrier per 64-thread wavefront (the whole block when smaller)
+  std::vector<pthread_barrier_t> wbar((bx + 63) / 64);
+  for (unsigned w = 0; w < wbar.size(); ++w)
+    pthread_barrier_init(&wbar[w], nullptr, bx % 64 && w + 1 == wbar.size() ? bx % 64 : 64);
+  g_wbar = &wbar;
+  std::vector<unsigned long long> xchg(bx);
+  pmx_emu_xchg = xchg.data();
   std::vector<pthread_t> th(bx);
   std::vector<Launch> L(bx);
   pthread_attr_t attr;
@@ -126,7 +139,10 @@ extern "C" int pmx_launch(void* function, unsigned gx, unsigned gy, unsigned gz,
   for (unsigned i = 0; i < bx; ++i) pthread_join(th[i], nullptr);
   pthread_attr_destroy(&attr);
   pthread_barrier_destroy(&bar);
+  for (auto& b : wbar) pthread_barrier_destroy(&b);
   g_bar = nullptr;
+  g_wbar = nullptr;
+  pmx_emu_xchg = nullptr;
   return 0;
 }
 

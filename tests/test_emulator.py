@@ -285,3 +285,34 @@ def _forwarded_asan(R, C, tiles, threads):
     g = np.full((3, R, C), -7.0, np.float32)
     kernel_emu.build(pipe, asan=True).run_host(dict(vals, mask=g))
     assert bits_equal(g, o)
+
+
+@pytest.mark.parametrize("app,R,C,tiles,threads,block", [
+    ('unsharp', 45, 77, [16, 60], 128, (8, 1)), ('unsharp', 37, 130, [16, 124], 256, (4, 1)),
+    ('unsharp', 16, 252, [16, 252], 512, (8, 1)), ('harris', 62, 70, [16, 32], 256, (8, 1)),
+    ('harris', 130, 517, [16, 128], 512, (8, 1)), ('harris', 5, 9, [4, 4], 64, (8, 1))])
+def test_emulated_lane_fusion(app, R, C, tiles, threads, block):
+    """gpu_options lanes=1: the producer sweep (unsharp's x-blur, Harris'
+    Sobel Ix/Iy) is computed in registers inside its consumer's sweep and
+    read across lanes with wavefront shuffles instead of through LDS;
+    partial tiles, the wavefront edge lanes and tiny images included.
+    Bit-exact against the oracle (the emulator runs __shfl on 64-thread
+    wavefronts)."""
+    pd = {}
+    out = unsharp_mask_full(pd) if app == 'unsharp' else harris_pipe(pd)
+    opts = {'threads': threads, 'block': block, 'lanes': 1}
+    if app == 'unsharp':
+        pipe = build(out, pd, {'R': R, 'C': C}, 'unsharp', tiles, group_size=5, gpu_options=opts)
+        img = inputs.smooth_field(np.random.default_rng(R + C), R + 4, C + 4, 3)
+        vals = {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0, 'img': img}
+        shape, oname, orc = (3, R, C), 'mask', 'unsharp_oracle_unsharp_mask_full'
+    else:
+        pipe = build(out, pd, {'R': R, 'C': C}, 'harris', tiles, gpu_options=opts)
+        vals = {'R': R, 'C': C, 'img': np.random.default_rng(R).random((R + 2, C + 2)).astype(np.float32)}
+        shape, oname, orc = (R + 2, C + 2), 'harris', 'harris_oracle_double'
+    assert 'pmx_shfl' in pipe.generate_code().hip_src
+    o = np.full(shape, -7.0, np.float32)
+    g = o.copy()
+    cemit.build([out], orc)(dict(vals, **{oname: o}))
+    kernel_emu.build(pipe).run_host(dict(vals, **{oname: g}))
+    assert bits_equal(g, o)
