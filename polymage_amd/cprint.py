@@ -166,4 +166,10 @@ __device__ __forceinline__ signed char pmx_shfl(signed char v, int src) { return
 __device__ __forceinline__ char pmx_shfl(char v, int src) { return (char)__shfl((int)v, src, 64); }
 __device__ __forceinline__ unsigned short pmx_shfl(unsigned short v, int src) { return (unsigned short)__shfl((int)v, src, 64); }
 __device__ __forceinline__ short pmx_shfl(short v, int src) { return (short)__shfl((int)v, src, 64); }
+#ifndef PMX_LD4
+#define PMX_LD4 1
+// four consecutive 4-byte elements at a 16-byte aligned address, one
+// 128-bit load (an LDS row run of a 4-wide block)
+template <typename V, typename T> __device__ __forceinline__ V pmx_ld4(const T* p) { return *(const V*)p; }
+#endif
 '''

@@ -58,6 +58,7 @@ class StageInfo(object):
         self.lds_class = None
         self.needs_lds = False
         self.lane = False               # kept in registers, read across lanes
+        self.vread = False              # read by a 4-wide block in 128-bit rows
 
     def reset(self):
         """Forget the analysis results (parts and live-out flag stay)."""
@@ -72,6 +73,7 @@ class StageInfo(object):
         self.lds_class = None
         self.needs_lds = False
         self.lane = False
+        self.vread = False
 
     @property
     def name(self):
@@ -786,7 +788,10 @@ class GroupPlan(object):
         straight-line code, so loads (and their conversions) shared by
         neighbouring points are issued once and consecutive loads merge
         into 128-bit accesses.  V = 4 only for stages that read no LDS
-        (lane-strided LDS accesses would conflict on banks); U and V
+        (lane-strided LDS accesses would conflict on banks), or, with the
+        gpu option ``vlds``, whose LDS reads are all row runs
+        (_vector_row_read): the block then loads each row it touches with
+        aligned 128-bit LDS reads, conflict-free; U and V
         minimise an estimate of thread time: sweep iterations times points
         per item times per-point cost, where the per-point load count
         shrinks with the reuse a block gives to the stage's stencil reach."""
@@ -814,10 +819,15 @@ class GroupPlan(object):
                 if i not in (rd, cd):
                     outer *= e
             reads_lds = False
+            vec_ok = bool(self.opts.get('vlds'))
+            lds_prods = []
             rv = rh = 0
             for (t, r) in self.closure_refs(s):
                 if self.in_group(r.objectRef) and not self.stage_of(r.objectRef).inline:
                     reads_lds = True
+                    p = self.stage_of(r.objectRef)
+                    lds_prods.append(p)
+                    vec_ok = vec_ok and self._vector_row_read(t, r, p, cd)
                 for arg in r.arguments:
                     sa = shift_access(arg, t.func.variables)
                     if sa is None or sa[0] == 'const' or not isinstance(sa[1], int):
@@ -826,13 +836,21 @@ class GroupPlan(object):
                         rh = max(rh, abs(sa[1]))
                     elif sa[0] == rd:
                         rv = max(rv, abs(sa[1]))
-            if forced is not None:
-                U, V = forced
+            lds_v = not reads_lds or vec_ok
+            own = forced
+            if reads_lds and self.opts.get('block_lds') is not None:
+                # (gpu option block_lds: the block of the stages that read LDS)
+                own = self.opts.get('block_lds')
+            if own is not None:
+                U, V = own
                 U = U if rd is not None else 1
-                V = V if (not reads_lds and ext_c >= 4 * V) else 1
+                V = V if (V in (1, 4) and lds_v and ext_c >= 4 * V) else 1
                 s.block = (max(1, min(U, ext_r)), V)
+                if V == 4:
+                    for p in lds_prods:
+                        p.vread = True
                 continue
-            Vs = [4, 1] if (not reads_lds and ext_c >= 16) else [1]
+            Vs = [4, 1] if (lds_v and ext_c >= 16) else [1]
             best = None
             for V in Vs:
                 for U in (1, 2, 4, 8):
@@ -846,7 +864,33 @@ class GroupPlan(object):
                     if best is None or key < best[0]:
                         best = (key, (U, V))
             s.block = best[1]
+            if s.block[1] == 4:
+                for p in lds_prods:
+                    p.vread = True
         self._sweeps()
+
+    def _vector_row_read(self, t, r, p, cd):
+        """True when the LDS read ``r`` (in stage t's variables) of producer
+        p can be served from 128-bit row loads by a 4-wide block: p holds
+        4-byte elements, r's innermost argument is the block's innermost
+        variable plus an integer constant, and its other arguments are
+        shifts of other variables (or constants), so each row the block
+        touches is a run of consecutive elements at a fixed offset."""
+        if cd is None or p.func.typ.nbytes() != 4 or p.lane:
+            return False
+        tv = t.func.variables
+        if len(r.arguments) != p.nd or p.nd == 0:
+            return False
+        for j, arg in enumerate(r.arguments):
+            sa = shift_access(arg, tv)
+            if sa is None:
+                return False
+            if j == p.nd - 1:
+                if sa[0] != cd or not isinstance(sa[1], int):
+                    return False
+            elif sa[0] == cd:
+                return False
+        return True
 
     def _sweeps(self):
         """Stages of one phase with the same static box, block and (on every
@@ -948,7 +992,7 @@ class GroupPlan(object):
         extent padded to a multiple of 4 for 4-wide blocks (aligned
         128-bit LDS stores)."""
         ext = list(s.static_ext)
-        if s.block[1] == 4:
+        if s.block[1] == 4 or s.vread:
             ext[-1] = (ext[-1] + 3) // 4 * 4
         return ext
 
@@ -976,6 +1020,10 @@ class GroupPlan(object):
             for e in self.lds_ext(s):
                 nbytes *= e
             nbytes = (nbytes + 15) // 16 * 16
+            if s.vread:
+                # a row load of a 4-wide block may run up to three elements
+                # past the last row (values no point uses)
+                nbytes += 16
             cls = (s.func.typ.c_name(), s.nd)
             s.lds_class = cls
             cands = sorted([fs for fs in free.get(cls, []) if fs[1] >= nbytes], key=lambda fs: fs[1])

@@ -260,6 +260,12 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             self.record = None
             self.forward = None
             self.lane_keys = None       # lane-fused reads this item makes
+            # 128-bit LDS row loads of a 4-wide block: a dry run collects the
+            # columns read per producer row (vprobe); the emitted fast path
+            # then reads components of the row registers (vmap)
+            self.vcd = None
+            self.vprobe = None
+            self.vmap = None
 
         def _base(self, key, make):
             if key not in self.names:
@@ -323,19 +329,25 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 p = plan.stage_of(f)
                 ext = plan.lds_ext(p)
                 W = [_prod(ext[j + 1:]) for j in range(len(ext))]
-
-                def mk(nm):
-                    terms = []
-                    for j, (k, off) in enumerate(sh):
-                        pbk = self.pb[k] if k != 'const' else '0'
-                        terms.append('__mul24(%s - %s, %d)' % (pbk, R(p.region[j][0]), W[j]))
-                    return 'const int %s = %s;' % (nm, ' + '.join(terms) if terms else '0')
-                nm = self._base(('l', p.name, kinds), mk)
+                nm = self.lds_base(p, kinds)
+                rows = []
                 for j, (k, off) in enumerate(sh):
                     o = self._dsum(k, off, pr) if k != 'const' else \
                         (off if isinstance(off, int) else pr.expr(off))
                     parts.append((o, W[j]))
+                    rows.append(o)
+                jc = len(sh) - 1
+                if p.vread and (self.vprobe is not None or self.vmap is not None) and \
+                        sh[jc][0] == self.vcd and isinstance(rows[jc], int):
+                    key = (p.name, kinds, tuple(rows[:jc]))
+                    if self.vprobe is not None:
+                        self.vprobe.setdefault(key, (p, set()))[1].add(rows[jc])
+                    elif key in self.vmap:
+                        cmin, prefix = self.vmap[key][:2]
+                        i = rows[jc] - cmin
+                        return '%s_%d.%s' % (prefix, i // 4, 'xyzw'[i % 4])
                 return 'L_%s[%s + %s]' % (p.name, nm, self._off(parts, pr))
+
             nd = len(sh)
 
             def mk(nm):
@@ -361,6 +373,31 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 if self.record is not None:
                     self.record.setdefault(key, load)
             return load
+
+        def lds_base(self, p, kinds):
+            """Name of the per-item LDS index of producer p's element at
+            the item's base point (consumer dimension ``kinds[j]`` for p's
+            dimension j, 'const' for a constant argument)."""
+            ext = plan.lds_ext(p)
+            W = [_prod(ext[j + 1:]) for j in range(len(ext))]
+
+            def mk(nm):
+                terms = []
+                for j, k in enumerate(kinds):
+                    pbk = self.pb[k] if k != 'const' else '0'
+                    terms.append('__mul24(%s - %s, %d)' % (pbk, R(p.region[j][0]), W[j]))
+                return 'const int %s = %s;' % (nm, ' + '.join(terms) if terms else '0')
+            return self._base(('l', p.name, kinds), mk)
+
+        def vrow_index(self, p, kinds, rows, col):
+            """LDS index of producer p's element at the given offsets from
+            the item's base point (``rows`` for every dimension but the
+            innermost, ``col`` for the innermost)."""
+            ext = plan.lds_ext(p)
+            W = [_prod(ext[j + 1:]) for j in range(len(ext))]
+            nm = self.lds_base(p, kinds)
+            parts = [(o, W[j]) for j, o in enumerate(rows)] + [(col, 1)]
+            return '%s + %s' % (nm, self._off(parts, None))
 
     # a tile whose region exceeds the planned static box (possible only for
     # sizes far from the planning estimates) is skipped and reported
@@ -637,7 +674,40 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 if key in fw_src:
                     fw_need.add(key)
             fitem.forward = dict((k, fw_vars[k]) for k in fw_need)
-        fast = []
+        vconds = []
+        vloads = []
+        if V == 4 and cd is not None and any(
+                plan.in_group(r.objectRef) and plan.stage_of(r.objectRef).vread
+                for m in members for (_, r) in plan.closure_refs(m)):
+            # the columns each producer row is read at over the block's
+            # points (a dry run), then one aligned 128-bit LDS load per four
+            # columns of each row; a block whose rows are not 16-byte
+            # aligned takes the per-point path
+            vprobe = Item(pb, dvec(0, 0))
+            vprobe.vcd = cd
+            vprobe.vprobe = {}
+            for m in members:
+                for (u, v) in pts:
+                    vprobe.d = dvec(u, v)
+                    emit_point([], m, vprobe, body_ind, 'x', 'y')
+            vmap = {}
+            for n, key in enumerate(sorted(vprobe.vprobe, key=str)):
+                p, cols = vprobe.vprobe[key]
+                cmin = min(cols)
+                nch = (max(cols) - cmin) // 4 + 1
+                vmap[key] = (cmin, 'lq%d' % n, nch, p)
+            fitem.vcd = cd
+            fitem.vmap = vmap
+            vtype = {'float': 'float4', 'int': 'int4', 'unsigned int': 'uint4'}
+            for key in sorted(vmap, key=lambda k: int(vmap[k][1][2:])):
+                cmin, prefix, nch, p = vmap[key]
+                idx = fitem.vrow_index(p, key[1], list(key[2]), cmin)
+                vt = vtype[c_type(p.func.typ)]
+                vconds.append('((%s) & 3) == 0' % idx)
+                for c in range(nch):
+                    vloads.append('  ' * body_ind + 'const %s %s_%d = pmx_ld4<%s>(&L_%s[%s + %d]);' % (
+                        vt, prefix, c, vt, p.name, idx, 4 * c))
+        fast = list(vloads)
         for mi, m in enumerate(members):
             T = c_type(m.func.typ)
             for (u, v) in pts:
@@ -691,6 +761,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                         fc.append(c)
             if fwd is not None and si == fwd[1] and fw_need:
                 fc.append('fw_ok')
+            fc.extend(vconds)
             w3('if (%s) {' % (' && '.join(fc) if fc else 'true'))
             SL.extend(fast)
             w3('} else {')
@@ -956,6 +1027,11 @@ def _forward_pair(plan, threads):
                 continue
             s2 = m2[0]
             if (s2.nd, s2.block, s2.bdims) != (s1.nd, s1.block, s1.bdims):
+                continue
+            if s2.block[1] == 4 and any(plan.in_group(r.objectRef) and plan.stage_of(r.objectRef).vread
+                                        for (t, r) in plan.closure_refs(s2)):
+                # its 128-bit LDS row loads align on its own item grid, not
+                # on the first sweep's (which starts at the producer's region)
                 continue
             g1 = set(r.objectRef for (t, r) in plan.closure_refs(s1) if not plan.in_group(r.objectRef))
             g2 = set(r.objectRef for (t, r) in plan.closure_refs(s2) if not plan.in_group(r.objectRef))

@@ -399,7 +399,7 @@ class Pipeline(object):
 
     # -- gather inlining ---------------------------------------------------------
     GATHER_OPS = 48
-    GATHER_REFS = 64
+    GATHER_REFS = 128
 
     def gather_inline_pass(self):
         """Substitute a function into the consumers that read it at a
@@ -416,9 +416,14 @@ class Pipeline(object):
         its other consumers keep reading
         it.  Repeats until nothing changes, so chains (the upsampled level
         read by the blend, the coarser level read by the upsampling)
-        collapse into reads of the coarser stored level at the two needed k."""
+        collapse into reads of the coarser stored level at the two needed k.
+        A reader is not grown past GATHER_REFS references (gpu option
+        ``gather_refs``): at 128 the local Laplacian's finest blend reads
+        the level-1 Gaussian planes directly instead of materialising the
+        upsampled level for all K values of k in the tile."""
         from .expr_ast import (AbstractBinaryOpNode, AbstractUnaryOpNode, Select,
                                InbuiltFunction, Cast)
+        max_refs = int((self._gpu_options or {}).get('gather_refs', self.GATHER_REFS))
         for _ in range(64):
             done = True
             for comp in list(self._comps):
@@ -475,7 +480,7 @@ class Pipeline(object):
                     # P's references along (a pyramid would otherwise
                     # unfold level by level into its finest level)
                     total = sum(len(case.collect(Reference)) for case in cf.defn)
-                    if m and total + len(m) * (prefs - 1) > self.GATHER_REFS:
+                    if m and total + len(m) * (prefs - 1) > max_refs:
                         m = {}
                     if m:
                         cf.replace_refs(m)

@@ -35,6 +35,20 @@ template <typename T> static inline T __shfl(T v, int src, int width = 64) {
   memcpy(&out, &r, sizeof(T));
   return out;
 }
+// HIP's 4-wide vector types as 16-byte aligned structs
+struct __attribute__((aligned(16))) float4 { float x, y, z, w; };
+struct __attribute__((aligned(16))) int4 { int x, y, z, w; };
+struct __attribute__((aligned(16))) uint4 { unsigned x, y, z, w; };
+// 128-bit loads must be 16-byte aligned on the device: a misaligned one is
+// counted (the harness fails the run) and served byte-wise
+extern "C" int pmx_emu_misaligned;
+#define PMX_LD4 1
+template <typename V, typename T> static inline V pmx_ld4(const T* p) {
+  V v;
+  if (((uintptr_t)p & 15u) != 0) __atomic_fetch_add(&pmx_emu_misaligned, 1, __ATOMIC_SEQ_CST);
+  memcpy(&v, p, sizeof(V));
+  return v;
+}
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 #define __builtin_amdgcn_readfirstlane(v) (v)
 #define PMX_GLOBAL

@@ -35,6 +35,7 @@ void __syncthreads() { pthread_barrier_wait(g_bar); }
 // lane exchange of a wavefront (__shfl): 64 threads share an exchange row
 // and meet at their own barrier
 unsigned long long* pmx_emu_xchg = nullptr;
+extern "C" { int pmx_emu_misaligned = 0; }
 static std::vector<pthread_barrier_t>* g_wbar = nullptr;
 void pmx_emu_wave_sync() { pthread_barrier_wait(&(*g_wbar)[threadIdx.x / 64]); }
 

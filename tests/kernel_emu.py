@@ -109,6 +109,10 @@ class EmulatedPipeline(object):
         self.outputs = pipe.sorted_outputs()
 
     def _check(self, rc, what):
+        bad = ctypes.c_int.in_dll(self.lib, 'pmx_emu_misaligned')
+        if bad.value:
+            n, bad.value = bad.value, 0
+            raise RuntimeError('%s: %d misaligned 128-bit loads' % (what, n))
         if rc != 0:
             raise RuntimeError('%s failed (%d): %s' % (what, rc,
                                                        self.lib.pmx_last_error().decode()))

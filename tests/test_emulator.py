@@ -316,3 +316,37 @@ def test_emulated_lane_fusion(app, R, C, tiles, threads, block):
     cemit.build([out], orc)(dict(vals, **{oname: o}))
     kernel_emu.build(pipe).run_host(dict(vals, **{oname: g}))
     assert bits_equal(g, o)
+
+
+@pytest.mark.parametrize("app,R,C,tiles,threads,block,fw", [
+    ('unsharp', 45, 77, [16, 60], 128, (2, 4), 0), ('unsharp', 37, 130, [16, 124], 256, (4, 4), 0),
+    ('unsharp', 16, 252, [16, 252], 512, (2, 4), 1), ('harris', 62, 70, [16, 32], 256, (4, 4), 0),
+    ('harris', 130, 517, [16, 128], 512, (2, 4), 0), ('harris', 5, 9, [4, 4], 64, (4, 4), 0)])
+def test_emulated_vector_lds_rows(app, R, C, tiles, threads, block, fw):
+    """gpu_options vlds=1 with a 4-wide block: a stage reading LDS loads
+    each producer row its block touches with aligned 128-bit reads
+    (``pmx_ld4``; the emulator fails the run on a misaligned one) and
+    takes its taps from those registers; blocks whose rows are not
+    aligned (with load forwarding the item grid starts at the producer's
+    region) and partial blocks take the per-point path.  Bit-exact
+    against the oracle."""
+    pd = {}
+    out = unsharp_mask_full(pd) if app == 'unsharp' else harris_pipe(pd)
+    opts = {'threads': threads, 'block': block, 'vlds': 1}
+    if not fw:
+        opts['no_forward'] = 1
+    if app == 'unsharp':
+        pipe = build(out, pd, {'R': R, 'C': C}, 'unsharp', tiles, group_size=5, gpu_options=opts)
+        img = inputs.smooth_field(np.random.default_rng(R + C), R + 4, C + 4, 3)
+        vals = {'R': R, 'C': C, 'threshold': 0.001, 'weight': 3.0, 'img': img}
+        shape, oname, orc = (3, R, C), 'mask', 'unsharp_oracle_unsharp_mask_full'
+    else:
+        pipe = build(out, pd, {'R': R, 'C': C}, 'harris', tiles, gpu_options=opts)
+        vals = {'R': R, 'C': C, 'img': np.random.default_rng(R).random((R + 2, C + 2)).astype(np.float32)}
+        shape, oname, orc = (R + 2, C + 2), 'harris', 'harris_oracle_double'
+    assert 'pmx_ld4' in pipe.generate_code().hip_src
+    o = np.full(shape, -7.0, np.float32)
+    g = o.copy()
+    cemit.build([out], orc)(dict(vals, **{oname: o}))
+    kernel_emu.build(pipe).run_host(dict(vals, **{oname: g}))
+    assert bits_equal(g, o)
