@@ -296,6 +296,47 @@ def shift_access(arg, consumer_vars):
     return (consumer_vars.index(v), off)
 
 
+def scaled_access(arg, consumer_vars):
+    """``(k, a, off)`` when ``arg`` is exactly ``a * consumer_vars[k] + off``
+    with integers a >= 1 and off (no truncating operator); ``('const', 0,
+    arg)`` for a parameter/constant argument; else None.  The shifts of
+    shift_access are the case a = 1 (with an integer offset); a = 2 is a
+    downsampling read (``f(2 * x + 1)``)."""
+    from .expression import substitute_vars, is_constant_expr, get_constant_from_expr
+    arg = Value.numericToValue(arg)
+    if arg.has(Reference) or arg.has(Select) or arg.has(Cast) or arg.has(InbuiltFunction):
+        return None
+    stack = [arg]
+    while stack:
+        e = stack.pop()
+        if isinstance(e, AbstractBinaryOpNode) and e.op in _NON_SHIFT_OPS:
+            return None
+        if hasattr(e, 'children'):
+            stack.extend(e.children())
+    allv = arg.collect(Variable) if arg.has(Variable) else []
+    used = [v for v in consumer_vars if v in allv]
+    others = [v for v in allv if v not in consumer_vars and not isinstance(v, Parameter)]
+    if others or len(used) > 1:
+        return None
+    if not used:
+        return ('const', 0, arg)
+    v = used[0]
+    if not isAffine(arg, include_div=False):
+        return None
+    coeff = get_affine_var_and_param_coeff(arg)
+    a = coeff.get(v)
+    if a is None or a != int(a) or int(a) < 1 or any(
+            k is not v and k != 1 and c for k, c in coeff.items()):
+        return None
+    off = Value.numericToValue(substitute_vars(arg, {v: Value(0, Int)}))
+    if not is_constant_expr(off):
+        return None
+    c = get_constant_from_expr(off)
+    if c != int(c):
+        return None
+    return (consumer_vars.index(v), int(a), int(c))
+
+
 def interval_image(acc, lo, hi, env):
     """Exact [min, max] of a monotone access over consumer values [lo, hi]
     (other variables bound in env)."""

@@ -94,11 +94,15 @@ def bilateral(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
 
 
 def laplacian(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=True,
-              group_size=5, K=8, J=4, literal_mode='double', inline=()):
+              group_size=5, K=8, J=4, literal_mode='double', inline=('colour',)):
     """Local Laplacian, uint16 RGB, K intensity levels, J pyramid levels,
     alpha = 1/(K-1), beta = 1; the input is padded by edge replication
     inside the pipeline so the output has the input's size.  ``inline``
-    names functions to inline into their readers (inline_directives)."""
+    names functions to inline into their readers (inline_directives): by
+    default the padded, widened input ``colour``, whose definition is a
+    cast of a clamped read of the input, so its readers read the input
+    directly (same values) instead of a padded copy written and read
+    back."""
     from .local_laplacian import local_laplacian
     from ..pipe import get_funcs
     pd = {}

@@ -31,7 +31,7 @@ import itertools
 import math
 
 from .analysis import (PlanError, classify_access, case_parts, ceval_int,
-                       eval_bexpr, shift_access)
+                       eval_bexpr, shift_access, scaled_access)
 from .constructs import Reference, Reduction, Variable
 from .region_ir import RegionIR, BIG
 
@@ -788,9 +788,9 @@ class GroupPlan(object):
         straight-line code, so loads (and their conversions) shared by
         neighbouring points are issued once and consecutive loads merge
         into 128-bit accesses.  V = 4 only for stages that read no LDS
-        (lane-strided LDS accesses would conflict on banks), or, with the
-        gpu option ``vlds``, whose LDS reads are all row runs
-        (_vector_row_read): the block then loads each row it touches with
+        (lane-strided LDS accesses would conflict on banks), or whose LDS
+        reads are all row runs (_vector_row_read; gpu option ``no_vlds``
+        turns this off): the block then loads each row it touches with
         aligned 128-bit LDS reads, conflict-free; U and V
         minimise an estimate of thread time: sweep iterations times points
         per item times per-point cost, where the per-point load count
@@ -819,7 +819,7 @@ class GroupPlan(object):
                 if i not in (rd, cd):
                     outer *= e
             reads_lds = False
-            vec_ok = bool(self.opts.get('vlds'))
+            vec_ok = not (self.opts.get('no_vlds') or self.opts.get('lanes'))
             lds_prods = []
             rv = rh = 0
             for (t, r) in self.closure_refs(s):
@@ -856,6 +856,11 @@ class GroupPlan(object):
                 for U in (1, 2, 4, 8):
                     if U > 1 and (rd is None or ext_r < 2 * U) or U * V > 16:
                         continue
+                    if V == 4 and reads_lds and U > 2:
+                        # the rows a 4-wide block loads from LDS stay in
+                        # registers: taller blocks measured slower (unsharp's
+                        # mask, Harris' sums: 2x4 best)
+                        continue
                     items = outer * (-(-ext_r // U)) * (-(-ext_c // V))
                     iters = -(-items // threads)
                     taps = (U + 2.0 * rv) / U * (V + 2.0 * rh) / V
@@ -884,7 +889,11 @@ class GroupPlan(object):
         for j, arg in enumerate(r.arguments):
             sa = shift_access(arg, tv)
             if sa is None:
-                return False
+                # a downsampling read (2 * y + c) of the row is a run too
+                sc = scaled_access(arg, tv)
+                if sc is None:
+                    return False
+                sa = (sc[0], sc[2])
             if j == p.nd - 1:
                 if sa[0] != cd or not isinstance(sa[1], int):
                     return False

@@ -23,7 +23,7 @@ reference, and ``pipeline_<name>_dev(..., stream)`` on device pointers.
 import hashlib
 import re
 
-from .analysis import PlanError, classify_access, eval_bexpr, unconditional_refs, shift_access
+from .analysis import PlanError, classify_access, eval_bexpr, unconditional_refs, shift_access, scaled_access
 from .constructs import Reference, Image, Case, Reduction
 from .cprint import Printer, DEVICE_PRELUDE, c_type
 from .group_plan import GroupPlan
@@ -220,6 +220,12 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                 sh = [shift_access(arg, cvars) for arg in ref.arguments]
                 if all(x is not None for x in sh):
                     return item.ref(f, sh, pr)
+                if plan.in_group(f) and not plan.stage_of(f).lane:
+                    # downsampling reads of an LDS stage (f(2 * x + 1)):
+                    # per-item base, constant offsets per point
+                    sc = [scaled_access(arg, cvars) for arg in ref.arguments]
+                    if all(x is not None for x in sc):
+                        return item.ref_scaled(f, sc, pr)
             idx = []
             for d, arg in enumerate(ref.arguments):
                 acc = classify_access(arg, cvars)
@@ -374,17 +380,53 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
                     self.record.setdefault(key, load)
             return load
 
+        def ref_scaled(self, f, sc, pr):
+            """An LDS read whose arguments are ``a * variable + c`` (a >= 1)
+            or constants: the per-item base is at ``a`` times the item's base
+            point, each point adds ``a * d + c`` per dimension."""
+            p = plan.stage_of(f)
+            ext = plan.lds_ext(p)
+            W = [_prod(ext[j + 1:]) for j in range(len(ext))]
+            kinds = tuple('const' if k == 'const' else (k if a == 1 else (k, a))
+                          for (k, a, c) in sc)
+            nm = self.lds_base(p, kinds)
+            parts = []
+            rows = []
+            for j, (k, a, c) in enumerate(sc):
+                if k == 'const':
+                    o = c if isinstance(c, int) else pr.expr(c)
+                else:
+                    d = self.d[k]
+                    o = a * d + c if isinstance(d, int) else '(%d * (%s) + %d)' % (a, d, c)
+                parts.append((o, W[j]))
+                rows.append(o)
+            jc = len(sc) - 1
+            if p.vread and (self.vprobe is not None or self.vmap is not None) and \
+                    sc[jc][0] == self.vcd and isinstance(rows[jc], int):
+                key = (p.name, kinds, tuple(rows[:jc]))
+                if self.vprobe is not None:
+                    self.vprobe.setdefault(key, (p, set()))[1].add(rows[jc])
+                elif key in self.vmap:
+                    cmin, prefix = self.vmap[key][:2]
+                    i = rows[jc] - cmin
+                    return '%s_%d.%s' % (prefix, i // 4, 'xyzw'[i % 4])
+            return 'L_%s[%s + %s]' % (p.name, nm, self._off(parts, pr))
+
         def lds_base(self, p, kinds):
             """Name of the per-item LDS index of producer p's element at
-            the item's base point (consumer dimension ``kinds[j]`` for p's
-            dimension j, 'const' for a constant argument)."""
+            the item's base point: consumer dimension ``kinds[j]`` (or
+            ``(k, a)``: a times dimension k) for p's dimension j, 'const'
+            for a constant argument."""
             ext = plan.lds_ext(p)
             W = [_prod(ext[j + 1:]) for j in range(len(ext))]
 
             def mk(nm):
                 terms = []
                 for j, k in enumerate(kinds):
-                    pbk = self.pb[k] if k != 'const' else '0'
+                    if isinstance(k, tuple):
+                        pbk = '%d * %s' % (k[1], self.pb[k[0]])
+                    else:
+                        pbk = self.pb[k] if k != 'const' else '0'
                     terms.append('__mul24(%s - %s, %d)' % (pbk, R(p.region[j][0]), W[j]))
                 return 'const int %s = %s;' % (nm, ' + '.join(terms) if terms else '0')
             return self._base(('l', p.name, kinds), mk)

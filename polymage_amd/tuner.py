@@ -223,6 +223,9 @@ class Space(object):
     applications with a lane-fusable group) lane fusion on or off."""
 
     def __init__(self, tiles, group_sizes, threads=(256,), lanes=(False,)):
+        self.tiles = list(tiles)
+        self.group_sizes = list(group_sizes)
+        self.threads = tuple(threads)
         self.candidates = [Candidate(t, g, w, l) for t, g, w, l in
                            itertools.product(tiles, group_sizes, threads, lanes)]
 
@@ -309,14 +312,14 @@ def default_space(app):
     """Tile sizes {8..512}^2 (powers of two, as the benchmark configuration
     states) and the same with the innermost size less the application's
     halo (``aligned_tiles``), times the application's group sizes times
-    workgroup sizes, times lane fusion on/off where the application's main
-    group has a lane-fusable pair of sweeps (unsharp, Harris).  Schedules
-    that do not fit (LDS) fail at planning and are skipped."""
+    workgroup sizes.  Schedules that do not fit (LDS) fail at planning and
+    are skipped.  Lane fusion (``lane_space``) is left out: on unsharp and
+    Harris its best candidates measured slower than the LDS schedules."""
     tiles = POW2_TILES + aligned_tiles(app)
     if app == 'unsharp':
-        return Space(tiles + lane_tiles(app), [1, 5], (128, 256, 512), lanes=(False, True))
+        return Space(tiles, [1, 5], (128, 256, 512))
     if app == 'harris':
-        return Space(tiles + lane_tiles(app), [1, 5, 11], (256, 512), lanes=(False, True))
+        return Space(tiles, [1, 5, 11], (256, 512))
     if app == 'bilateral':
         return Space(tiles, [1, 5], (128, 256))
     if app == 'laplacian':
@@ -328,6 +331,15 @@ def default_space(app):
     if app == 'blend':
         return Space(tiles, [1, 3, 5, 6], (256,))
     raise KeyError(app)
+
+
+def lane_space(app):
+    """Lane-fused candidates (gpu option ``lanes``) of an application whose
+    main group has a lane-fusable pair of sweeps: the default tiles plus
+    ``lane_tiles``, with lane fusion on."""
+    base = default_space(app)
+    return Space(list(base.tiles) + lane_tiles(app), list(base.group_sizes),
+                 tuple(base.threads), lanes=(True,))
 
 
 def build_candidate(workload_fn, cand, with_data=False):

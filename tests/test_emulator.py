@@ -350,3 +350,19 @@ def test_emulated_vector_lds_rows(app, R, C, tiles, threads, block, fw):
     cemit.build([out], orc)(dict(vals, **{oname: o}))
     kernel_emu.build(pipe).run_host(dict(vals, **{oname: g}))
     assert bits_equal(g, o)
+
+
+@pytest.mark.parametrize("tiles,opts", [([16, 32], {'block_lds': (1, 4)}),
+                                        ([16, 32], {'block_lds': (2, 4)}),
+                                        ([32, 64], {'block': (1, 4), 'block_lds': (2, 4)}),
+                                        ([8, 16], {'block_lds': (2, 4), 'threads': 64})])
+def test_emulated_laplacian_downsampling_rows(monkeypatch, tiles, opts):
+    """The local Laplacian's downsampling stages read their LDS producer
+    at 2 * x + c (rows) and 2 * y + c (columns): in 4-wide blocks those
+    reads come from 128-bit row loads (per-item base at twice the item's
+    point); the padded input inlined, gather-inlined blends.  Bit-exact
+    against the oracle, every 128-bit load aligned."""
+    import test_gpu_parity as P
+    monkeypatch.setattr(P, 'gpu_build', kernel_emu.build)
+    g, o = P._laplacian(96, 80, tiles, gpu_options=opts, inline=['colour'], group_size=8)
+    assert np.array_equal(g, o)

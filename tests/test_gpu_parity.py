@@ -139,14 +139,15 @@ def test_bilateral_full_size(gpu):
     assert bits_equal(g, o)
 
 
-def _laplacian(Rin, Cin, tiles, K=8, J=4, io='u16', mode='double', seed=9, group_size=5):
+def _laplacian(Rin, Cin, tiles, K=8, J=4, io='u16', mode='double', seed=9, group_size=5,
+               gpu_options=None, inline=None):
     from polymage_amd.apps.local_laplacian import local_laplacian
     pd = {}
     out = local_laplacian(pd, K=K, J=J, io=io, lut_hi_inclusive=(io == 'u16'))
     lo, hi = pd['pad']
     R, C = Rin + lo + hi, Cin + lo + hi
     pipe = build(out, pd, {'R': R, 'C': C}, 'laplacian', tiles, group_size=group_size,
-                 literal_mode=mode)
+                 literal_mode=mode, gpu_options=gpu_options, inline=inline)
     g_ = np.random.default_rng(seed)
     if io == 'u16':
         img = np.ascontiguousarray((inputs.smooth_field(g_, Rin, Cin, 3) * 65535).round()
