@@ -31,6 +31,46 @@ def c_type(t):
     return t.c_name()
 
 
+# value ranges of the integer types narrow enough for an exhaustive check
+_NARROW = {'unsigned char': (0, 255), 'char': (-128, 127), 'signed char': (-128, 127),
+           'unsigned short': (0, 65535), 'short': (-32768, 32767)}
+_DIVC = {}
+
+
+def narrow_int_range(e):
+    """(lo, hi) when expression e is a read of a function or image, or a
+    cast, of an integer type of at most 16 bits (its C value lies in the
+    type's range even after promotion), else None."""
+    if isinstance(e, (Reference, Cast)):
+        t = e.objectRef.typ if isinstance(e, Reference) else e.typ
+        return _NARROW.get(t.c_name())
+    return None
+
+
+def divc_exact(d, lo, hi):
+    """True when, for every integer n in [lo, hi], the double quotient
+    n / d (IEEE division, round to nearest) equals q + (n - q*d) * r with
+    r = RN(1/d), q = RN(n*r) and both corrections fused multiply-adds
+    (one Newton step).  Checked exhaustively in exact rational arithmetic,
+    so the rewrite is used only where it gives the division's bits."""
+    from fractions import Fraction
+    key = (d, lo, hi)
+    if key not in _DIVC:
+        ok = d != 0 and d == d and abs(d) != float('inf')
+        if ok:
+            r = 1.0 / d
+            fd, fr = Fraction(d), Fraction(r)
+            for n in range(lo, hi + 1):
+                q = n * r
+                e = float(n - Fraction(q) * fd)
+                q1 = float(Fraction(e) * fr + Fraction(q))
+                if q1 != n / d or (q1 == 0 and str(q1) != str(n / d)):
+                    ok = False
+                    break
+        _DIVC[key] = ok
+    return _DIVC[key]
+
+
 def literal(v, mode):
     if isinstance(v.value, float):
         if mode == 'float12' and v.typ is Float:
@@ -70,6 +110,16 @@ class Printer(object):
         if isinstance(e, Variable):
             return self.var_names[e]
         if isinstance(e, AbstractBinaryOpNode):
+            if e.op == '/' and self.mode == 'double':
+                # a narrow integer over a double literal (u16 / 65535.0):
+                # a multiply and two FMAs give the division's exact bits
+                # (checked for every value of the integer type)
+                d = Value.numericToValue(e.right)
+                rng = narrow_int_range(e.left)
+                if rng is not None and isinstance(d, Value) and isinstance(d.value, float) \
+                        and divc_exact(d.value, rng[0], rng[1]):
+                    return 'pmx_divc((double)(%s), %r, %r)' % (
+                        self.expr(e.left, pro), 1.0 / d.value, d.value)
             return '(' + self.expr(e.left, pro) + ' ' + e.op + ' ' + \
                 self.expr(e.right, pro) + ')'
         if isinstance(e, AbstractUnaryOpNode):
@@ -166,6 +216,13 @@ __device__ __forceinline__ signed char pmx_shfl(signed char v, int src) { return
 __device__ __forceinline__ char pmx_shfl(char v, int src) { return (char)__shfl((int)v, src, 64); }
 __device__ __forceinline__ unsigned short pmx_shfl(unsigned short v, int src) { return (unsigned short)__shfl((int)v, src, 64); }
 __device__ __forceinline__ short pmx_shfl(short v, int src) { return (short)__shfl((int)v, src, 64); }
+// n / d for n an integer of at most 16 bits, as q + (n - q*d) / d with
+// q = n * (1/d): callers pass r = RN(1/d) only where this is checked to
+// equal the IEEE quotient for every such n
+__device__ __forceinline__ double pmx_divc(double n, double r, double d) {
+  const double q = n * r;
+  return __builtin_fma(__builtin_fma(-q, d, n), r, q);
+}
 #ifndef PMX_LD4
 #define PMX_LD4 1
 // four consecutive 4-byte elements at a 16-byte aligned address, one

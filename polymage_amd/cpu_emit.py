@@ -83,6 +83,10 @@ static inline int pmx_imin(int a, int b) { return a < b ? a : b; }
 static inline int pmx_imax(int a, int b) { return a > b ? a : b; }
 static inline int pmx_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 static inline int pmx_fdiv(int a, int k) { return a >= 0 ? a / k : -((-a + k - 1) / k); }
+static inline double pmx_divc(double n, double r, double d) {
+  const double q = n * r;
+  return std::fma(std::fma(-q, d, n), r, q);
+}
 #define PMX_BIG 536870912
 '''
 

@@ -39,6 +39,10 @@ LDS_LIMIT_BYTES = 160 * 1024
 CHEAP_OPS = 4          # operations of a pointwise stage recomputed per tap
 
 
+# longest leading dimension kept whole in a tile (gpu option kblock)
+KBLOCK_MAX = 16
+
+
 class StageInfo(object):
     def __init__(self, comp, index):
         self.comp = comp
@@ -401,6 +405,11 @@ class GroupPlan(object):
                 self.grid_dims.append((d, int(self.tile_sizes[k])))
                 k += 1
             elif d < best.nd - 1 and self._identity_dim(d):
+                if self.opts.get('kblock') and d == 0 and best.nd >= 3 and ext <= KBLOCK_MAX:
+                    # (gpu option kblock) a short leading dimension (the
+                    # local Laplacian's K levels) stays whole in the tile;
+                    # threads then block over it (_blocks)
+                    continue
                 # (never the innermost dimension: splitting it would leave
                 # each workgroup a strided, uncoalesced slice of the rows)
                 self.grid_dims.append((d, 1))
@@ -812,6 +821,14 @@ class GroupPlan(object):
                 # the two innermost dimensions, blocks along the third
                 rd = s.nd - 3
                 s.bdims = (rd, cd)
+            elif self._kblock_dim(s):
+                # a whole short leading dimension: each thread computes all
+                # of its points for one (row, column), so what they share
+                # (loads, index arithmetic, a gray value's table index) is
+                # done once
+                s.bdims = (0, cd)
+                s.block = (ext[0], 1)
+                continue
             ext_c = ext[cd]
             ext_r = ext[rd] if rd is not None else 1
             outer = 1
@@ -873,6 +890,17 @@ class GroupPlan(object):
                 for p in lds_prods:
                     p.vread = True
         self._sweeps()
+
+    def _kblock_dim(self, s):
+        """True when (gpu option kblock) stage s's leading dimension is a
+        whole, unsplit, short dimension of the tile (at most KBLOCK_MAX
+        points) and s has at least three dimensions."""
+        if not self.opts.get('kblock') or s.nd < 3 or s.align[0] is None:
+            return False
+        ad = s.align[0][0]
+        if any(d == ad for (d, T) in self.grid_dims):
+            return False
+        return 2 <= s.static_ext[0] <= KBLOCK_MAX
 
     def _vector_row_read(self, t, r, p, cd):
         """True when the LDS read ``r`` (in stage t's variables) of producer

@@ -100,6 +100,8 @@ class ConstPlan(object):
         w('static inline double pmx_sin(double v) { return std::sin(v); }')
         w('static inline float pmx_cos(float v) { return std::cos(v); }')
         w('static inline double pmx_cos(double v) { return std::cos(v); }')
+        w('static inline double pmx_divc(double n, double r, double d) '
+          '{ const double q = n * r; return std::fma(std::fma(-q, d, n), r, q); }')
         w('#endif')
         sig = ', '.join('%s %s' % (c_type(p.typ), p.name) for p in self.params)
         used = set(id(q) for q in f.getObjects(Parameter))

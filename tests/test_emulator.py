@@ -366,3 +366,17 @@ def test_emulated_laplacian_downsampling_rows(monkeypatch, tiles, opts):
     monkeypatch.setattr(P, 'gpu_build', kernel_emu.build)
     g, o = P._laplacian(96, 80, tiles, gpu_options=opts, inline=['colour'], group_size=8)
     assert np.array_equal(g, o)
+
+
+@pytest.mark.parametrize("tiles,opts", [([8, 16], {'kblock': 1}), ([16, 32], {'kblock': 1}),
+                                        ([8, 32], {'kblock': 1, 'threads': 128})])
+def test_emulated_laplacian_level_blocks(monkeypatch, tiles, opts):
+    """gpu option kblock: the K intensity levels stay whole in each tile
+    and every thread computes all K levels of its (row, column), sharing
+    the gray value, its table index and the index arithmetic; partial
+    tiles included.  Bit-exact against the oracle."""
+    import test_gpu_parity as P
+    from polymage_amd.hip_emit import plan_pipeline
+    monkeypatch.setattr(P, 'gpu_build', kernel_emu.build)
+    g, o = P._laplacian(96, 80, tiles, gpu_options=opts, inline=['colour'], group_size=8)
+    assert np.array_equal(g, o)
