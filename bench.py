@@ -422,6 +422,54 @@ def app_entry(args, torch, dev, app, cpu, headline=None):
     return ent
 
 
+def classed_batch(args, torch, w, recs, outs, refs, stream):
+    """The same frames split into the autotuner's size classes, each class
+    through its own compiled pipeline (the tiles tuned on a frame of that
+    size): one persistent launch per class, and the classes' captured
+    graphs.  Every frame is compared with the oracle after each mode.
+    None when no classes are recorded."""
+    import numpy as np
+    from polymage_amd import runtime
+    from polymage_amd.apps import workloads
+    from polymage_amd.size_classes import ClassedBatch
+    which = args.schedule if args.schedule in ('learned', 'exhaustive') else 'learned'
+    sched = workloads.class_schedules('campipe', which)
+    if not sched:
+        return None
+    built = []
+    for (max_area, tiles, gs, gopts, label) in sched:
+        cw = workloads.campipe(tiles=tiles, group_size=gs, gpu_options=gopts, with_data=False)
+        built.append(runtime.build(cw.pipe))
+    classes = [{'max_area': a} for (a, _, _, _, _) in sched]
+    cb = ClassedBatch(built, classes, recs, lambda r: r['R'] * r['C'])
+    res, parity = {}, {}
+    try:
+        for mode in ('persistent', 'graph'):
+            for _ in range(args.warmup):
+                cb.run(stream, mode)
+            torch.cuda.synchronize()
+            for d in outs:
+                d.fill_(7)
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            for _ in range(args.steps):
+                cb.run(stream, mode)
+            ev1.record()
+            torch.cuda.synchronize()
+            res[mode] = ev0.elapsed_time(ev1) / args.steps
+            st = cb.status()
+            bad = [i for i, (d, r) in enumerate(zip(outs, refs)) if not np.array_equal(d.cpu().numpy(), r)]
+            parity[mode] = 'bitexact' if (st == 0 and not bad) else \
+                'MISMATCH (status %d, frames %s)' % (st, bad[:8])
+    finally:
+        cb.destroy()
+    return {'classes': [{'max_area': s[0], 'schedule': s[4], 'frames': len(g),
+                         'code_key': b.code.key} for s, g, b in zip(sched, cb.groups, built)],
+            'persistent_ms': round(res['persistent'], 4), 'graph_ms': round(res['graph'], 4),
+            'parity': parity}
+
+
 def batch_entry(args, torch, dev, cpu):
     """48 camera-pipeline frames of uneven sizes (uint16 Bayer in, uint8 RGB
     out): one persistent work-queue launch per kernel over all frames, the
@@ -493,6 +541,7 @@ def batch_entry(args, torch, dev, cpu):
             torch.cuda.current_stream().synchronize()
     res['frames_sync'] = (time.perf_counter() - t0) * 1e3 / reps
     bp.batch_destroy(h)
+    classed = classed_batch(args, torch, w, recs, outs, refs, stream)
     ms = res['graph']
     achieved = ideal / (ms / 1e3) / 1e9
     ent = {'schedule': w.schedule, 'frames': len(frames), 'mpix': round(mpix, 3), 'ideal_bytes': ideal,
@@ -505,6 +554,10 @@ def batch_entry(args, torch, dev, cpu):
            'speedup_persistent_vs_frames_sync': round(res['frames_sync'] / res['persistent'], 3),
            'frac': round(achieved / HBM_PEAK_GBS, 4), 'achieved_gbs': round(achieved, 2),
            'parity': parity, 'recorded_tuning': recorded_tuning('campipe')}
+    if classed is not None:
+        classed['speedup_persistent_vs_frames'] = round(res['frames'] / classed['persistent_ms'], 3)
+        classed['speedup_graph_vs_frames'] = round(res['frames'] / classed['graph_ms'], 3)
+        ent['size_classes'] = classed
     if not args.no_cpu:
         try:
             omp = _omp()

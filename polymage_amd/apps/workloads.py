@@ -204,6 +204,10 @@ def prebuild_default():
     for name in ('harris', 'unsharp', 'bilateral', 'laplacian', 'campipe', 'interpolate', 'blend'):
         for which in ('default', 'learned', 'exhaustive'):
             runtime.build(tuned(name, which, with_data=False).pipe)
+    for which in ('learned', 'exhaustive'):
+        for (_, tiles, gs, gopts, _) in class_schedules('campipe', which):
+            runtime.build(campipe(tiles=tiles, group_size=gs, gpu_options=gopts,
+                                  with_data=False).pipe)
 
 
 BUILDERS = {'harris': harris, 'unsharp': unsharp, 'bilateral': bilateral,
@@ -224,6 +228,31 @@ def schedule(app, which='learned'):
     if rec.get('lanes'):
         gopts['lanes'] = 1
     return tuple(rec['tiles']), rec['group_size'], gopts, '%s: %s' % (which, rec['key'])
+
+
+def class_schedules(app, which='learned'):
+    """The frame-size classes recorded for ``app`` by the autotuner
+    (``autotune --size-classes``): a list of (max_area, tiles, group_size,
+    gpu_options, label), smallest class first, or [] when none."""
+    import json
+    import os
+    from .. import tuner
+    if not os.path.exists(tuner.TUNED_PATH):
+        return []
+    with open(tuner.TUNED_PATH) as f:
+        rec = json.load(f).get(app) or {}
+    out = []
+    for c in rec.get('classes', []):
+        key = c.get('learned_best' if which == 'learned' else 'exhaustive_best')
+        if not key:
+            return []
+        k = tuner.parse_key(key)
+        gopts = {'threads': k['threads']}
+        if k.get('lanes'):
+            gopts['lanes'] = 1
+        out.append((c['max_area'], tuple(k['tiles']), k['group_size'], gopts,
+                    '%s %dx%d: %s' % (which, c['rows'], c['cols'], key)))
+    return out
 
 
 def tuned(app, which='learned', **kw):

@@ -36,20 +36,47 @@ import time
 import numpy as np
 
 
+def _split(app):
+    """'campipe@608x832' -> ('campipe', {'rows': 608, 'cols': 832}): an
+    application tuned on a frame of a given size (a size class)."""
+    if '@' not in app:
+        return app, {}
+    name, size = app.split('@')
+    r, c = size.split('x')
+    return name, {'rows': int(r), 'cols': int(c)}
+
+
 def _workload(app):
     from .apps import workloads
-    return getattr(workloads, app)
+    return getattr(workloads, _split(app)[0])
+
+
+# size-class specs -> the frames of the class: such a spec is timed as one
+# persistent launch over the class's frames (the batch it will run), not
+# as a single frame's plain launch
+_CLASS_FRAMES = {}
+
+
+def _time(app, comp, runs, db, verbose=True):
+    from . import tuner
+    name, kw = _split(app)
+    if app in _CLASS_FRAMES:
+        return tuner.time_candidates_batch(_workload(app), comp, _CLASS_FRAMES[app],
+                                           db=db, app=app, verbose=verbose)
+    return tuner.time_candidates(_workload(app), comp, runs=runs, db=db, app=app,
+                                 verbose=verbose, **kw)
 
 
 def exhaustive(app, procs, db, runs=9, pool=None):
     from . import tuner
-    cands = list(tuner.default_space(app))
-    tuner.workload_data(_workload(app))
+    name, kw = _split(app)
+    cands = list(tuner.default_space(name))
+    tuner.workload_data(_workload(app), **kw)
     t0 = time.time()
-    comp = tuner.compile_all(_workload(app), cands, procs=procs, verbose=True, pool=pool)
+    comp = tuner.compile_all(_workload(app), cands, procs=procs, verbose=True, pool=pool, **kw)
     t_compile = time.time() - t0
     t1 = time.time()
-    times = tuner.time_candidates(_workload(app), comp, runs=runs, db=db, app=app, verbose=True)
+    times = _time(app, comp, runs, db)
     t_time = time.time() - t1
     feats = dict((k, v[1]) for k, v in comp.items() if v[1] is not None)
     errors = dict((k, v[3]) for k, v in comp.items() if v[3])
@@ -65,8 +92,9 @@ def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None
     split of its wall-clock time."""
     from . import tuner, costmodel
     t0 = time.time()
-    cands = list(tuner.default_space(app))
-    feats = tuner.features_all(_workload(app), cands, procs=procs, pool=pool)
+    name, kw = _split(app)
+    cands = list(tuner.default_space(name))
+    feats = tuner.features_all(_workload(app), cands, procs=procs, pool=pool, **kw)
     feats = dict((key, f) for key, f in feats.items() if f is not None)
     t_feat = time.time() - t0
     keys = list(feats)
@@ -96,10 +124,9 @@ def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None
         tops.append(top_keys)
         top = [c for c in cands if c.key in top_keys]
         t2 = time.time()
-        comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True, pool=pool)
+        comp = tuner.compile_all(_workload(app), top, procs=procs, verbose=True, pool=pool, **kw)
         t3 = time.time()
-        times.update(tuner.time_candidates(_workload(app), comp, runs=runs, db=None, app=app,
-                                           verbose=True))
+        times.update(_time(app, comp, runs, None))
         t_gpu += time.time() - t3
         t_top += time.time() - t2
     return {'times': times, 'top': [kk for t in tops for kk in t], 'rounds': tops,
@@ -110,8 +137,28 @@ def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None
 def _default_key(app):
     from .apps.workloads import DEFAULT_TILES
     from .tuner import Candidate
+    app = _split(app)[0]
     gs = {'harris': 11, 'campipe': 40}.get(app, 5)
     return Candidate(DEFAULT_TILES.get(app, (32, 64)), gs, 256).key
+
+
+def size_class_specs(spec):
+    """'campipe:3' -> (app, classes, ['campipe@RxC', ...]): the size classes
+    of the application's benchmark batch (the camera pipeline's 48 seeded
+    frames) and the spec of each class's representative frame.  Each
+    class's candidates are planned at the representative size and timed as
+    a persistent batch launch over the class's own frames."""
+    from .apps import inputs
+    from .size_classes import frame_classes
+    app, k = spec.split(':')
+    if app != 'campipe':
+        raise ValueError('size classes are defined for the campipe batch only')
+    frames = inputs.campipe_frames(48, 5)
+    classes = frame_classes([f.shape for f in frames], int(k))
+    specs = ['%s@%dx%d' % (app, c['rows'], c['cols']) for c in classes]
+    for spec, c in zip(specs, classes):
+        _CLASS_FRAMES[spec] = [frames[i] for i in c['members']]
+    return app, classes, specs
 
 
 def main():
@@ -124,6 +171,8 @@ def main():
     ap.add_argument('--runs', type=int, default=9)
     ap.add_argument('--reuse-db', default=None, help='timings.jsonl from an earlier run')
     ap.add_argument('--record', action='store_true', help='write polymage_amd/tuned.json')
+    ap.add_argument('--size-classes', default=None,
+                    help="APP:K: also tune each of the K frame-size classes of APP's batch")
     args = ap.parse_args()
     from .tuner import TimingDB
     os.makedirs(args.out, exist_ok=True)
@@ -132,7 +181,12 @@ def main():
     from . import runtime
     runtime.CACHE = os.environ['PMX_KCACHE']
     db = TimingDB(os.path.join(args.out, 'timings.jsonl'))
-    apps = args.apps.split(',')
+    apps = [a for a in args.apps.split(',') if a]
+    class_info = None
+    if args.size_classes:
+        capp, classes, specs = size_class_specs(args.size_classes)
+        class_info = (capp, classes, specs)
+        apps += specs
     from .tuner import make_pool
     pool = make_pool(args.procs)
     ex = {}
@@ -149,14 +203,15 @@ def main():
             records += [json.loads(l) for l in f if l.strip()]
     report = {}
     for app in apps:
-        train = [r for r in records if r['app'] != app and r.get('features')]
+        # (never the same application, at any frame size)
+        train = [r for r in records if _split(r['app'])[0] != _split(app)[0] and r.get('features')]
         print('learned', app, 'trained on', len(train), 'records', flush=True)
         # fresh cache again so the learned path pays its own compiles
         runtime.CACHE = tempfile.mkdtemp(prefix='pmx_kc_')
         le = learned(app, args.procs, train, k=args.k, rounds=args.rounds, runs=args.runs,
                      pool=pool)
         t_train = le['train_s']
-        w = _workload(app)(with_data=False)
+        w = _workload(app)(with_data=False, **_split(app)[1])
         ex_times = ex[app]['times']
         ex_best_key, ex_best = min(ex_times.items(), key=lambda kv: kv[1])
         le_best_key, le_best = min(le['times'].items(), key=lambda kv: kv[1])
@@ -200,6 +255,12 @@ def main():
             json.dump(report, f, indent=1)
     pool.close()
     pool.join()
+    if class_info is not None:
+        capp, classes, specs = class_info
+        report['%s:classes' % capp] = [dict((k, v) for k, v in c.items() if k != 'members')
+                                       for c in classes]
+        with open(os.path.join(args.out, 'tuning.json'), 'w') as f:
+            json.dump(report, f, indent=1)
     if args.record:
         record(report)
 
@@ -212,11 +273,25 @@ def record(report, path=None):
     if os.path.exists(path):
         with open(path) as f:
             keep = json.load(f)
+    fields = ('exhaustive_best', 'exhaustive_ms', 'learned_best', 'learned_ms',
+              'learned_over_exhaustive', 'learned_over_exhaustive_same_run',
+              'tuning_time_ratio', 'timed_ratio', 'gpu_time_ratio', 'candidates', 'timed')
     for app, r in report.items():
-        keep[app] = dict((k, r[k]) for k in (
-            'exhaustive_best', 'exhaustive_ms', 'learned_best', 'learned_ms',
-            'learned_over_exhaustive', 'learned_over_exhaustive_same_run',
-            'tuning_time_ratio', 'timed_ratio', 'gpu_time_ratio', 'candidates', 'timed'))
+        if app.endswith(':classes') or '@' in app:
+            continue
+        keep[app] = dict(keep.get(app, {}), **dict((k, r[k]) for k in fields))
+    # size classes: the class bounds and each class's picks, kept with the
+    # application (its single-size picks stay as they are)
+    for app, classes in report.items():
+        if not app.endswith(':classes'):
+            continue
+        name = app.split(':')[0]
+        out = []
+        for c in classes:
+            spec = '%s@%dx%d' % (name, c['rows'], c['cols'])
+            if spec in report:
+                out.append(dict(c, **dict((k, report[spec][k]) for k in fields)))
+        keep.setdefault(name, {})['classes'] = out
     with open(path, 'w') as f:
         json.dump(keep, f, indent=1, sort_keys=True)
 

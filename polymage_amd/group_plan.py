@@ -893,12 +893,14 @@ class GroupPlan(object):
 
     def _kblock_dim(self, s):
         """True when (gpu option kblock) stage s's leading dimension is a
-        whole, unsplit, short dimension of the tile (at most KBLOCK_MAX
+        whole, unsplit, short dimension of the tile (2 to KBLOCK_MAX
         points) and s has at least three dimensions."""
-        if not self.opts.get('kblock') or s.nd < 3 or s.align[0] is None:
+        if not self.opts.get('kblock') or s.nd < 3:
             return False
-        ad = s.align[0][0]
-        if any(d == ad for (d, T) in self.grid_dims):
+        # (a leading dimension its consumers read at constants, like the
+        # bilateral grid's value/weight pair, is aligned with no anchor
+        # dimension and is whole in every tile)
+        if s.align[0] is not None and any(d == s.align[0][0] for (d, T) in self.grid_dims):
             return False
         return 2 <= s.static_ext[0] <= KBLOCK_MAX
 

@@ -321,7 +321,7 @@ def default_space(app):
     if app == 'harris':
         return Space(tiles, [1, 5, 11], (256, 512))
     if app == 'bilateral':
-        return Space(tiles, [1, 5], (128, 256))
+        return Space(tiles, [1, 5], (128, 256, 512))
     if app == 'laplacian':
         return Space(tiles, [1, 3, 5, 8], (256,))
     if app == 'campipe':
@@ -529,6 +529,74 @@ def time_candidates(workload, compiled, runs=9, db=None, app=None, verbose=False
         if db is not None:
             db.add({'app': app, 'key': key, 'ms': ms, 'mpix': w.mpix, 'features': feats,
                     'compile_s': dt})
+    return out
+
+
+def time_batch(bp, workload, frames, frame_input='img', runs=5, warmup=1, reps=3):
+    """Milliseconds per persistent work-queue launch of ``bp`` over a batch
+    of frames (``frames``: input arrays of the workload's ``frame_input``,
+    of any sizes; the workload function gives every other value and the
+    output shape at each frame's size), device-resident; median of ``reps``
+    batches of ``runs`` launches between HIP events."""
+    import numpy as np
+    import torch
+    dev = torch.device('cuda:0')
+    recs = []
+    for img in frames:
+        R, C = img.shape[:2]
+        w = workload(rows=R, cols=C, with_data=False)
+        rec = dict((k, v) for k, v in w.values.items() if not hasattr(v, 'shape'))
+        for f in bp.inputs:
+            a = img if f.name == frame_input else w.values[f.name]
+            rec[f.name] = torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).to(dev)
+        for f in bp.outputs:
+            rec[f.name] = torch.empty(int(np.prod(w.out_shape)) * f.typ.nbytes(),
+                                      dtype=torch.uint8, device=dev)
+        recs.append(rec)
+    h = bp.batch_create(recs)
+    try:
+        stream = torch.cuda.current_stream().cuda_stream
+        for _ in range(warmup):
+            bp.batch_run(h, stream, 'persistent')
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(runs):
+                bp.batch_run(h, stream, 'persistent')
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) / runs)
+    finally:
+        bp.batch_destroy(h)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def time_candidates_batch(workload, compiled, frames, runs=5, db=None, app=None, verbose=False):
+    """time_candidates with a batch of frames as the objective (the
+    persistent launch over all of them, ``time_batch``); returns {key: ms}."""
+    w = workload(rows=frames[0].shape[0], cols=frames[0].shape[1], with_data=False)
+    mpix = 0.0
+    out = {}
+    for key, (so, feats, dt, err) in compiled.items():
+        if so is None:
+            continue
+        bp = runtime.load(so, w.pipe)
+        try:
+            ms = time_batch(bp, workload, frames, runs=runs)
+        except Exception as e:
+            if db is not None:
+                db.add({'app': app, 'key': key, 'error': str(e)[:200]})
+            continue
+        out[key] = ms
+        if verbose:
+            print('  timed    %-22s %8.4f ms (batch of %d)' % (key, ms, len(frames)), flush=True)
+        if db is not None:
+            db.add({'app': app, 'key': key, 'ms': ms, 'mpix': mpix, 'features': feats,
+                    'compile_s': dt, 'batch': len(frames)})
     return out
 
 

@@ -91,7 +91,8 @@ def _records():
 def test_tuning_records_match_the_feature_set():
     from polymage_amd.costmodel import FEATURES
     recs = _records()
-    assert len(set(r['app'] for r in recs)) == 5 and len(recs) > 500
+    # the five benchmark pipelines and the two other reference apps
+    assert len(set(r['app'] for r in recs)) == 7 and len(recs) > 500
     assert all(set(r['features']) >= set(FEATURES) and r['ms'] > 0 for r in recs)
 
 
@@ -106,7 +107,7 @@ def _expected_best_of_random(ms, n):
 @pytest.mark.gpu
 def test_learned_search_gains_over_search_without_the_model(gpu):
     """Replay of the learned search on the recorded GPU timings of the
-    five applications (tuning_records.jsonl, measured by the autotuner on
+    seven tuned applications (tuning_records.jsonl, measured by the autotuner on
     an MI355X): for each application, an ensemble of HIP-trained MLPs that
     never saw it ranks its schedules, the top 10 are "timed" (looked up),
     the ensemble is refitted with them and the next 10 are timed.  The 20

@@ -380,3 +380,14 @@ def test_emulated_laplacian_level_blocks(monkeypatch, tiles, opts):
     monkeypatch.setattr(P, 'gpu_build', kernel_emu.build)
     g, o = P._laplacian(96, 80, tiles, gpu_options=opts, inline=['colour'], group_size=8)
     assert np.array_equal(g, o)
+
+
+@pytest.mark.parametrize("rows,cols,tiles", [(100, 77, [16, 32]), (256, 192, [32, 64])])
+def test_emulated_bilateral_pair_blocks(monkeypatch, rows, cols, tiles):
+    """gpu option kblock on the bilateral grid: the (value, weight) pair of
+    the slice is one block, so both read the pixel, its grid cell and
+    weights once.  Bit-exact against the oracle."""
+    import test_gpu_parity as P
+    monkeypatch.setattr(P, 'gpu_build', kernel_emu.build)
+    g, o = P._bilateral(rows, cols, tiles, gpu_options={'kblock': 1})
+    assert bits_equal(g, o)
