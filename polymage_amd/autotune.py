@@ -84,12 +84,14 @@ def exhaustive(app, procs, db, runs=9, pool=None):
             'compile_s': t_compile, 'time_s': t_time, 'n': len(cands)}
 
 
-def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None):
+def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None, seed_k=5):
     """The learned search: rank every candidate by the cost model trained on
-    the other applications' timings, compile and time the top k, refit with
-    those k measurements of this application (weighted by repetition), time
-    the next k, and keep the fastest measured.  Returns timings and the
-    split of its wall-clock time."""
+    the other applications' timings, compile and time the top k (of which
+    the first ``seed_k`` are the analytic estimate's best: the prior the
+    model corrects), refit with those k measurements of this application
+    (weighted by repetition), time the model's next k, and keep the
+    fastest measured.  Returns timings and the split of its wall-clock
+    time."""
     from . import tuner, costmodel
     t0 = time.time()
     name, kw = _split(app)
@@ -120,6 +122,13 @@ def learned(app, procs, train_records, k=10, rounds=2, runs=9, dup=20, pool=None
         pred = model.predict(Xc)
         t_train += time.time() - t1
         order = [keys[i] for i in np.argsort(pred) if keys[i] not in times]
+        if rnd == 0 and seed_k:
+            # the first round also times the analytic estimate's best
+            # seed_k (a roofline prior: exact when one resource bounds the
+            # schedule), the model fills the rest of the round
+            est = np.array([feats[kk]['log_est_ns_px'] for kk in keys])
+            seeds = [keys[i] for i in np.argsort(est)][:seed_k]
+            order = seeds + [kk for kk in order if kk not in seeds]
         top_keys = order[:k]
         tops.append(top_keys)
         top = [c for c in cands if c.key in top_keys]
