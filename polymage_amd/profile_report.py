@@ -42,6 +42,48 @@ def counter(d, kernel, name):
     return [float(r['Counter_Value']) for r in rows]
 
 
+SQ_NAMES = ['SQ_WAVES', 'SQ_INSTS_VALU', 'SQ_ACTIVE_INST_VALU', 'SQ_INSTS_SALU', 'SQ_INSTS_SMEM',
+            'SQ_INSTS_VMEM_RD', 'SQ_INSTS_VMEM_WR', 'SQ_INSTS_LDS', 'SQ_LDS_BANK_CONFLICT',
+            'SQ_BUSY_CYCLES', 'SQ_WAVE_CYCLES', 'SQ_WAIT_INST_ANY', 'GRBM_GUI_ACTIVE']
+# CUs of one MI355X and its peak engine clock: a CU issues at most one
+# wave64 VALU instruction per cycle (four 16-lane SIMDs, four cycles each)
+N_CU = 256
+CLOCK_MHZ = 2400.0
+
+
+def sq_table(dirs, prefix, kernel_us):
+    """Per-kernel means per dispatch of the SQ/GRBM counters in the --pmc
+    runs ``dirs``, for kernels whose name starts with ``prefix``, with two
+    derived columns: VALU busy (SQ_INSTS_VALU / (CUs x cycles of the traced
+    average duration at the peak clock)) and LDS bank-conflict cycles per
+    LDS instruction."""
+    vals = {}
+    for d in dirs:
+        for r in _csv(d, '_counter_collection.csv'):
+            k = r['Kernel_Name'].split('(')[0]
+            if not k.startswith(prefix) or r['Counter_Name'] not in SQ_NAMES:
+                continue
+            vals.setdefault(k, {}).setdefault(r['Counter_Name'], []).append(float(r['Counter_Value']))
+    if not vals:
+        return []
+    have = [n for n in SQ_NAMES if any(n in v for v in vals.values())]
+    L = ['SQ counters, mean per dispatch (`--pmc`, counters summed over the GPU):', '',
+         '| kernel | us | ' + ' | '.join(n.replace('SQ_', '') for n in have) +
+         ' | VALU busy | conflict cycles / LDS instr |',
+         '|---|---|' + '---|' * len(have) + '---|---|']
+    for k in sorted(vals):
+        v = dict((n, statistics.mean(x)) for n, x in vals[k].items())
+        us = kernel_us.get(k)
+        busy = (v['SQ_INSTS_VALU'] / (N_CU * us * CLOCK_MHZ)) if (us and 'SQ_INSTS_VALU' in v) else None
+        conf = (v['SQ_LDS_BANK_CONFLICT'] / v['SQ_INSTS_LDS']) if v.get('SQ_INSTS_LDS') else None
+        L.append('| %s | %s | %s | %s | %s |' % (
+            k, '%.2f' % us if us else '-', ' | '.join('%.4g' % v[n] if n in v else '-' for n in have),
+            '%.2f' % busy if busy is not None else '-', '%.3f' % conf if conf is not None else '-'))
+    L += ['', 'VALU busy = SQ_INSTS_VALU / (%d CUs x traced average duration x %.0f MHz): a CU '
+              'issues at most one wave64 VALU instruction per cycle.' % (N_CU, CLOCK_MHZ), '']
+    return L
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--kernel', required=True)
@@ -52,7 +94,8 @@ def main():
     ap.add_argument('--copy-fetch')
     ap.add_argument('--copy-write')
     ap.add_argument('--copy-mib', type=float, default=512)
-    ap.add_argument('--sq')
+    ap.add_argument('--sq', action='append', default=[],
+                    help='a --pmc run of SQ/GRBM counters (may be repeated)')
     ap.add_argument('--ideal', type=float, default=None, help='ideal bytes per dispatch')
     ap.add_argument('--ideal-in', type=float, default=None, help='input bytes per dispatch')
     ap.add_argument('--commands', default='', help='text: the commands that made the data')
@@ -70,8 +113,11 @@ def main():
     if args.commands:
         L += ['Commands (repository root, one MI355X):', '', '```', args.commands.strip(), '```', '']
     avg_us = None
+    kernel_us = {}
     if args.trace:
         rows = _csv(args.trace, '_kernel_stats.csv')
+        for r in rows:
+            kernel_us[r['Name']] = float(r['AverageNs']) / 1e3
         L += ['Kernel statistics (`--kernel-trace --stats`):', '',
               '| kernel | calls | average (us) | min (us) | max (us) | % |', '|---|---|---|---|---|---|']
         per_call = 0.0
@@ -123,18 +169,7 @@ def main():
                 avg_us, (args.ideal or 0) / avg_us / 1e3, traffic / avg_us / 1e3))
         L.append('')
     if args.sq:
-        names = ['SQ_WAVES', 'SQ_INSTS_VALU', 'SQ_INSTS_SALU', 'SQ_INSTS_SMEM', 'SQ_INSTS_VMEM_RD',
-                 'SQ_INSTS_VMEM_WR', 'SQ_INSTS_LDS', 'SQ_LDS_BANK_CONFLICT', 'SQ_ACTIVE_INST_VALU',
-                 'SQ_BUSY_CYCLES', 'SQ_WAVE_CYCLES', 'GRBM_GUI_ACTIVE']
-        L += ['SQ counters per dispatch (one `--pmc` run):', '', '| counter | value |', '|---|---|']
-        for n in names:
-            try:
-                v = counter(args.sq, args.kernel, n)
-            except (FileNotFoundError, ValueError):
-                continue
-            if v:
-                L.append('| %s | %.0f |' % (n, statistics.mean(v)))
-        L.append('')
+        L += sq_table(args.sq, args.kernel, kernel_us)
     with open(args.out, 'w') as fo:
         fo.write('\n'.join(L) + '\n')
     if args.traffic_json and traffic is not None:
