@@ -42,6 +42,17 @@ def counter(d, kernel, name):
     return [float(r['Counter_Value']) for r in rows]
 
 
+def kernel_means(d, prefix, name):
+    """{kernel: mean per dispatch of counter ``name``} for the kernels whose
+    name starts with ``prefix``."""
+    vals = {}
+    for r in _csv(d, '_counter_collection.csv'):
+        k = r['Kernel_Name'].split('(')[0]
+        if r['Counter_Name'] == name and k.startswith(prefix):
+            vals.setdefault(k, []).append(float(r['Counter_Value']))
+    return dict((k, statistics.mean(v)) for k, v in vals.items())
+
+
 SQ_NAMES = ['SQ_WAVES', 'SQ_INSTS_VALU', 'SQ_ACTIVE_INST_VALU', 'SQ_INSTS_SALU', 'SQ_INSTS_SMEM',
             'SQ_INSTS_VMEM_RD', 'SQ_INSTS_VMEM_WR', 'SQ_INSTS_LDS', 'SQ_LDS_BANK_CONFLICT',
             'SQ_BUSY_CYCLES', 'SQ_WAVE_CYCLES', 'SQ_WAIT_INST_ANY', 'GRBM_GUI_ACTIVE']
@@ -100,8 +111,9 @@ def main():
     ap.add_argument('--ideal-in', type=float, default=None, help='input bytes per dispatch')
     ap.add_argument('--commands', default='', help='text: the commands that made the data')
     ap.add_argument('--calls', type=int, default=0,
-                    help='pipeline calls in the PMC runs: sum the matching dispatches per call '
-                         '(pipelines of several kernels); 0 = average per dispatch')
+                    help='pipelines of several kernels: traffic per pipeline call; N > 0 sums '
+                         'the matching dispatches over N calls, -1 sums each matching kernel\'s '
+                         'mean per dispatch (every kernel runs once per call); 0 = average per dispatch')
     ap.add_argument('--out', required=True)
     ap.add_argument('--traffic-json')
     ap.add_argument('--code-key')
@@ -137,7 +149,11 @@ def main():
         L.append('')
     traffic = None
     if args.fetch and args.write and args.copy_fetch and args.copy_write:
-        if args.calls:
+        if args.calls < 0:
+            # multi-kernel pipeline: each kernel's mean per dispatch, summed
+            f = sum(kernel_means(args.fetch, args.kernel, 'FETCH_SIZE').values())
+            wr = sum(kernel_means(args.write, args.kernel, 'WRITE_SIZE').values())
+        elif args.calls:
             # multi-kernel pipeline: all matching dispatches per pipeline call
             f = sum(counter(args.fetch, args.kernel, 'FETCH_SIZE')) / args.calls
             wr = sum(counter(args.write, args.kernel, 'WRITE_SIZE')) / args.calls

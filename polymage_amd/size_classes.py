@@ -68,12 +68,14 @@ class ClassedBatch(object):
             self.destroy()
             raise
 
-    def run(self, stream=0, mode='persistent'):
+    def run(self, stream=0, mode='persistent', streams=None):
         """Every class's frames, one launch (or graph) per class, in class
-        order on ``stream``."""
-        for bp, h in zip(self.built, self.handles):
+        order on ``stream``; with ``streams`` (one stream handle per class)
+        class i goes to streams[i], so the classes' launches can overlap
+        (ordering against other work on those streams is the caller's)."""
+        for i, (bp, h) in enumerate(zip(self.built, self.handles)):
             if h is not None:
-                bp.batch_run(h, stream, mode)
+                bp.batch_run(h, streams[i] if streams else stream, mode)
 
     def status(self):
         """The first nonzero device status of the class pipelines, or 0."""
