@@ -71,6 +71,56 @@ def divc_exact(d, lo, hi):
     return _DIVC[key]
 
 
+# significand bits of every value of a type that converts to double exactly
+# (float's 24 with its exponent range; integers of at most 16 bits)
+_EXACT_BITS = dict([('float', 24)] + [(t, 16) for t in _NARROW])
+
+
+def _exact_bits(e):
+    """Significand bits of the C value of e when e is a read, cast or
+    parameter of a type in _EXACT_BITS (the printed text then has that C
+    type: function reads print as typed loads or typed variables)."""
+    if isinstance(e, Reference):
+        t = e.objectRef.typ
+    elif isinstance(e, (Cast, Parameter)):
+        t = e.typ
+    else:
+        return None
+    return _EXACT_BITS.get(t.c_name())
+
+
+def _const_bits(v):
+    """Significand bits of a finite nonzero double constant whose binary
+    exponent lies within +-800 (so its product with a float or a narrow
+    integer neither overflows nor leaves double's normal range), else
+    None."""
+    import math
+    if not isinstance(v, float) or v == 0 or v != v or abs(v) == float('inf'):
+        return None
+    if abs(math.frexp(v)[1]) > 800:
+        return None
+    num, _ = abs(v).as_integer_ratio()
+    while num % 2 == 0:
+        num //= 2
+    return num.bit_length()
+
+
+def exact_product(e):
+    """(a, k) when e is a * k (either order) with k a double literal and
+    a an expression whose value has few enough significand bits that a * k
+    is exact in double (bits(a) + bits(k) <= 53), else None.  Then
+    l + a * k rounds once, exactly as fma(a, k, l) does."""
+    if not (isinstance(e, AbstractBinaryOpNode) and e.op == '*'):
+        return None
+    for a, c in ((e.left, e.right), (e.right, e.left)):
+        cv = Value.numericToValue(c)
+        if isinstance(cv, Value) and isinstance(cv.value, float):
+            ab, cb = _exact_bits(a), _const_bits(cv.value)
+            if ab is not None and cb is not None and ab + cb <= 53:
+                return a, cv.value
+    return None
+
+
 def literal(v, mode):
     if isinstance(v.value, float):
         if mode == 'float12' and v.typ is Float:
@@ -89,12 +139,16 @@ class Printer(object):
     """Prints expressions; ``ref_cb(ref, printer, prologue)`` returns the C
     text of a Reference, ``var_names`` maps Variables to C names."""
 
-    def __init__(self, var_names, ref_cb, mode='double', param_prefix='', lazy=False):
+    def __init__(self, var_names, ref_cb, mode='double', param_prefix='', lazy=False,
+                 exact_fma=False):
         self.var_names = var_names
         self.ref_cb = ref_cb
         self.mode = mode
         self.param_prefix = param_prefix
         self.lazy = lazy
+        # l +- a * k as one fused multiply-add where the product is exact
+        # (exact_product): the same bits as the separate multiply and add
+        self.exact_fma = exact_fma and mode == 'double'
         self._tmp = [0]
 
     def tmp(self):
@@ -120,6 +174,10 @@ class Printer(object):
                         and divc_exact(d.value, rng[0], rng[1]):
                     return 'pmx_divc((double)(%s), %r, %r)' % (
                         self.expr(e.left, pro), 1.0 / d.value, d.value)
+            if self.exact_fma and e.op in ('+', '-'):
+                f = self._fma(e, pro)
+                if f is not None:
+                    return f
             return '(' + self.expr(e.left, pro) + ' ' + e.op + ' ' + \
                 self.expr(e.right, pro) + ')'
         if isinstance(e, AbstractUnaryOpNode):
@@ -173,6 +231,21 @@ class Printer(object):
             raise NotImplementedError("RandomFloat has no GPU lowering")
         raise TypeError(type(e))
 
+    def _fma(self, e, pro):
+        """l + a*k, a*k + l, l - a*k or a*k - l with an exact product a*k,
+        as fma(a, +-k, +-l): RN(l + p) is what both forms compute."""
+        r = exact_product(e.right)
+        if r is not None:
+            a, k = r
+            return 'pmx_fmad((double)(%s), %r, (double)(%s))' % (
+                self.expr(a, pro), k if e.op == '+' else -k, self.expr(e.left, pro))
+        r = exact_product(e.left)
+        if r is not None:
+            a, k = r
+            return 'pmx_fmad((double)(%s), %r, %s(double)(%s))' % (
+                self.expr(a, pro), k, '' if e.op == '+' else '-', self.expr(e.right, pro))
+        return None
+
     def cond(self, c, pro=None):
         if c.conditional in ('&&', '||'):
             return '(' + self.cond(c.lhs, pro) + ' ' + c.conditional + ' ' + \
@@ -219,6 +292,7 @@ __device__ __forceinline__ short pmx_shfl(short v, int src) { return (short)__sh
 // n / d for n an integer of at most 16 bits, as q + (n - q*d) / d with
 // q = n * (1/d): callers pass r = RN(1/d) only where this is checked to
 // equal the IEEE quotient for every such n
+__device__ __forceinline__ double pmx_fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ double pmx_divc(double n, double r, double d) {
   const double q = n * r;
   return __builtin_fma(__builtin_fma(-q, d, n), r, q);

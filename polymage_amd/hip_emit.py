@@ -487,7 +487,8 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         ``target`` and setting ``pmx_w`` when a part applied."""
         ww = lambda t, i: out.append('  ' * i + t)
         vn = dict(zip(s.func.variables, pnames))
-        pr = Printer(vn, make_ref_cb(s, item), mode, lazy=True)
+        pr = Printer(vn, make_ref_cb(s, item), mode, lazy=True,
+                     exact_fma=not plan.opts.get('no_exact_fma'))
         pr._tmp = tmp_counter
         single = (len(s.parts) == 1)
         for pi, part in enumerate(s.parts):
