@@ -121,6 +121,174 @@ def exact_product(e):
     return None
 
 
+def same_expr(a, b):
+    """Structural equality of two expressions (or conditions): the same
+    operators over the same variables, objects and typed constants."""
+    a, b = Value.numericToValue(a), Value.numericToValue(b)
+    if isinstance(a, Condition) or isinstance(b, Condition):
+        return isinstance(a, Condition) and isinstance(b, Condition) and \
+            a.conditional == b.conditional and same_expr(a.lhs, b.lhs) and same_expr(a.rhs, b.rhs)
+    if type(a) is not type(b):
+        return False
+    if isinstance(a, Value):
+        return a.typ == b.typ and repr(a.value) == repr(b.value)
+    if isinstance(a, Variable):
+        return a is b
+    if isinstance(a, AbstractBinaryOpNode):
+        return a.op == b.op and same_expr(a.left, b.left) and same_expr(a.right, b.right)
+    if isinstance(a, AbstractUnaryOpNode):
+        return a.op == b.op and same_expr(a.child, b.child)
+    if isinstance(a, Reference):
+        return a.objectRef is b.objectRef and len(a.arguments) == len(b.arguments) and \
+            all(same_expr(x, y) for x, y in zip(a.arguments, b.arguments))
+    if isinstance(a, Cast):
+        return a.typ == b.typ and same_expr(a.expression, b.expression)
+    if isinstance(a, Select):
+        return same_expr(a.condition, b.condition) and \
+            same_expr(a.true_expression, b.true_expression) and \
+            same_expr(a.false_expression, b.false_expression)
+    if isinstance(a, InbuiltFunction):
+        return len(a.arguments) == len(b.arguments) and \
+            all(same_expr(x, y) for x, y in zip(a.arguments, b.arguments))
+    return False
+
+
+_RANK = {'int': 1, 'unsigned int': 2, 'long long': 3, 'unsigned long long': 4,
+         'float': 5, 'double': 6}
+
+
+def _promote(t):
+    return 'int' if t in _NARROW else t
+
+
+class CSelect(Select):
+    """``cond ? a : b`` of two expressions whose printed C text has the same
+    type ``ctype``: printed as a plain conditional (only the taken branch
+    is evaluated), so the value keeps that C type."""
+
+    def __init__(self, cond, a, b, ctype):
+        Select.__init__(self, cond, a, b, False)
+        self.ctype = ctype
+
+
+class SelectUnifier(object):
+    """An expression equal, point by point, to ``cond ? a : b`` in which the
+    selection is pushed down to where the two branches differ: a and b
+    built by the same operators over the same C types share every node
+    above their differences, so, e.g., the 0.25/0.75 upsampling of two
+    neighbours computes its arithmetic once and reads one of the two
+    differing neighbours instead of both.  The C type of each printed node
+    is tracked (double literals make it differ from the DSL type), and a
+    node is shared only where both branches' nodes have the same C type,
+    so a point performs exactly the arithmetic of its taken branch.
+    ``cheap_ref(r)``: a reference whose load is cheap (a constant offset
+    from a per-item base) is kept whole, two loads selected, rather than
+    selecting its index."""
+
+    def __init__(self, mode, var_names, cheap_ref=None):
+        self.mode = mode
+        self.var_names = var_names
+        self.cheap_ref = cheap_ref
+
+    def ctype(self, e):
+        """C type of e's printed text, or None when not tracked."""
+        e = Value.numericToValue(e)
+        if isinstance(e, CSelect):
+            return e.ctype
+        if isinstance(e, Value):
+            if isinstance(e.value, float):
+                return 'float' if (self.mode == 'float12' and e.typ is Float) else 'double'
+            return 'int' if -2 ** 31 <= e.value < 2 ** 31 else None
+        if isinstance(e, Parameter):
+            return _promote(e.typ.c_name())
+        if isinstance(e, Variable):
+            return 'int' if e in self.var_names else None
+        if isinstance(e, Reference):
+            return _promote(e.objectRef.typ.c_name())
+        if isinstance(e, Cast):
+            return _promote(e.typ.c_name())
+        if isinstance(e, Select):
+            # printed with a cast to the select's type
+            return _promote(c_type(getType(e)))
+        if isinstance(e, AbstractUnaryOpNode):
+            return self.ctype(e.child) if e.op in ('-', '+') else None
+        if isinstance(e, AbstractBinaryOpNode):
+            if e.op == '/' and self.mode == 'double' and isinstance(
+                    Value.numericToValue(e.right), Value) and narrow_int_range(e.left):
+                return None     # may print as pmx_divc
+            l, r = self.ctype(e.left), self.ctype(e.right)
+            if l is None or r is None or l not in _RANK or r not in _RANK:
+                return None
+            if e.op in ('<<', '>>'):
+                return l
+            return l if _RANK[l] >= _RANK[r] else r
+        return None
+
+    def index_only(self, u):
+        """True when every selection left in u chooses between two pure
+        arithmetic expressions (indices, constants: no reads, calls or
+        selects), which compile to a conditional move; merging is then
+        worth it.  Otherwise the two branches are kept."""
+        stack = [u]
+        while stack:
+            e = Value.numericToValue(stack.pop())
+            if isinstance(e, CSelect):
+                for x in (e.true_expression, e.false_expression):
+                    x = Value.numericToValue(x)
+                    if x.has(Reference) or x.has(Select) or x.has(InbuiltFunction):
+                        return False
+                continue
+            if isinstance(e, Select):
+                stack += [e.true_expression, e.false_expression]
+            elif hasattr(e, 'children'):
+                stack += list(e.children())
+        return True
+
+    def unify(self, cond, a, b):
+        """The unified expression, or None when a and b have different
+        (or untracked) C types."""
+        a, b = Value.numericToValue(a), Value.numericToValue(b)
+        if same_expr(a, b):
+            return a
+        ta, tb = self.ctype(a), self.ctype(b)
+        if ta is None or ta != tb:
+            return None
+        leaf = CSelect(cond, a, b, ta)
+        if type(a) is not type(b):
+            return leaf
+        if isinstance(a, AbstractBinaryOpNode) and a.op == b.op:
+            l = self.unify(cond, a.left, b.left)
+            r = self.unify(cond, a.right, b.right)
+            if l is None or r is None:
+                return leaf
+            if type(a) is AbstractBinaryOpNode:
+                return AbstractBinaryOpNode(l, r, a.op)
+            return type(a)(l, r)
+        if isinstance(a, Cast) and a.typ == b.typ:
+            x = self.unify(cond, a.expression, b.expression)
+            return leaf if x is None else Cast(a.typ, x)
+        if isinstance(a, Reference) and a.objectRef is b.objectRef and \
+                len(a.arguments) == len(b.arguments):
+            if self.cheap_ref is not None and self.cheap_ref(a) and self.cheap_ref(b):
+                return leaf
+            args = [self.unify(cond, x, y) for x, y in zip(a.arguments, b.arguments)]
+            if any(x is None for x in args):
+                return leaf
+            r = Reference(a.objectRef, args)
+            # the references it stands for (their per-reference decisions,
+            # such as clamping, apply to it)
+            r.unified_from = getattr(a, 'unified_from', (a,)) + getattr(b, 'unified_from', (b,))
+            return r
+        if type(a) is Select and same_expr(a.condition, b.condition):
+            t = self.unify(cond, a.true_expression, b.true_expression)
+            f = self.unify(cond, a.false_expression, b.false_expression)
+            if t is None or f is None:
+                return leaf
+            # printed with a cast to the select's (shared) type
+            return Select(a.condition, t, f, False)
+        return leaf
+
+
 def literal(v, mode):
     if isinstance(v.value, float):
         if mode == 'float12' and v.typ is Float:
@@ -140,7 +308,7 @@ class Printer(object):
     text of a Reference, ``var_names`` maps Variables to C names."""
 
     def __init__(self, var_names, ref_cb, mode='double', param_prefix='', lazy=False,
-                 exact_fma=False):
+                 exact_fma=False, unify=None):
         self.var_names = var_names
         self.ref_cb = ref_cb
         self.mode = mode
@@ -149,6 +317,10 @@ class Printer(object):
         # l +- a * k as one fused multiply-add where the product is exact
         # (exact_product): the same bits as the separate multiply and add
         self.exact_fma = exact_fma and mode == 'double'
+        # lazy Selects whose branches share structure print through
+        # SelectUnifier when only index arithmetic is left to select;
+        # ``unify`` is its cheap_ref test (None: off)
+        self.unify = unify
         self._tmp = [0]
 
     def tmp(self):
@@ -185,6 +357,15 @@ class Printer(object):
         if isinstance(e, Reference):
             return self.ref_cb(e, self, pro)
         if isinstance(e, Select):
+            if isinstance(e, CSelect):
+                return '((%s) ? (%s) : (%s))' % (self.cond(e.condition, pro),
+                                                 self.expr(e.true_expression, pro),
+                                                 self.expr(e.false_expression, pro))
+            if self.lazy and pro is not None and self.unify is not None and type(e) is Select:
+                uf = SelectUnifier(self.mode, self.var_names, self.unify)
+                u = uf.unify(e.condition, e.true_expression, e.false_expression)
+                if u is not None and not isinstance(u, CSelect) and uf.index_only(u):
+                    return '((%s)(%s))' % (c_type(getType(e)), self.expr(u, pro))
             if self.lazy and pro is not None:
                 # Same value as the reference's "both branches into typed
                 # temporaries, then select", but only the taken branch runs,

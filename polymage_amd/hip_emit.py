@@ -213,7 +213,11 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         def cb(ref, pr, pro):
             f = ref.objectRef
             cvars = cur_stage.func.variables
-            clamped = any((id(ref), d) in plan.clamp_refs for d in range(len(ref.arguments)))
+            # a reference merged from the two branches of a select
+            # (SelectUnifier) is clamped where either of them is
+            rids = [id(o) for o in getattr(ref, 'unified_from', (ref,))]
+            cl = lambda d: any((i, d) in plan.clamp_refs for i in rids)
+            clamped = any(cl(d) for d in range(len(ref.arguments)))
             if item is not None and not clamped and \
                     not (plan.in_group(f) and plan.stage_of(f).inline) and \
                     (plan.in_group(f) or id(ref) in uncond):
@@ -230,7 +234,7 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
             for d, arg in enumerate(ref.arguments):
                 acc = classify_access(arg, cvars)
                 q = pr.expr(arg, pro)
-                if acc.kind == '*' or (id(ref), d) in plan.clamp_refs or \
+                if acc.kind == '*' or cl(d) or \
                         (id(ref) not in uncond and not plan.in_group(f)):
                     if plan.in_group(f):
                         p = plan.stage_of(f)
@@ -487,8 +491,16 @@ def emit_group_kernel(pipe, plan, kname, threads, mode, xcd=True):
         ``target`` and setting ``pmx_w`` when a part applied."""
         ww = lambda t, i: out.append('  ' * i + t)
         vn = dict(zip(s.func.variables, pnames))
+        cv = s.func.variables
+
+        def cheap(r):
+            # loads at constant offsets from a per-item base stay whole
+            # when the branches of a select are merged
+            return all(shift_access(a, cv) is not None or scaled_access(a, cv) is not None
+                       for a in r.arguments)
         pr = Printer(vn, make_ref_cb(s, item), mode, lazy=True,
-                     exact_fma=not plan.opts.get('no_exact_fma'))
+                     exact_fma=not plan.opts.get('no_exact_fma'),
+                     unify=None if plan.opts.get('no_unify') else cheap)
         pr._tmp = tmp_counter
         single = (len(s.parts) == 1)
         for pi, part in enumerate(s.parts):
