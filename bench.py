@@ -278,15 +278,16 @@ class DeviceRun(object):
         for d in self.sets:
             d[self.out.name].copy_(t)
 
-    # untimed launches before the first measurement of the process, so the
-    # GPU clock has left its idle level (once per process, before warmup)
-    SETTLE_S = 0.05
+    # untimed launches before the first measurement of each run, so the
+    # GPU clock has left the idle level it drops to while the host works
+    # (CPU baselines, parity checks between pipelines), before warmup
+    SETTLE_S = 0.25
     settled = False
 
     def time(self, steps, warmup):
         torch = self.torch
-        if not DeviceRun.settled:
-            DeviceRun.settled = True
+        if not self.settled:
+            self.settled = True
             t_end = time.perf_counter() + DeviceRun.SETTLE_S
             k = 0
             while time.perf_counter() < t_end:
@@ -543,6 +544,12 @@ def batch_entry(args, torch, dev, cpu):
     res = {}
     parity = {}
     for mode in ('frames', 'persistent', 'graph'):
+        # each mode from a settled clock (the oracle checks above leave the
+        # GPU idle for seconds)
+        t_end = time.perf_counter() + DeviceRun.SETTLE_S
+        while time.perf_counter() < t_end:
+            bp.batch_run(h, stream, mode)
+            torch.cuda.synchronize()
         for _ in range(args.warmup):
             bp.batch_run(h, stream, mode)
         torch.cuda.synchronize()
