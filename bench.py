@@ -358,7 +358,10 @@ def build_workload(args, app, which, base=None):
     gopts = {}
     for kv in args.gpu_opt:
         k, v = kv.split('=', 1)
-        gopts[k] = tuple(int(x) for x in v.split(',')) if ',' in v else int(v)
+        if v[:1] in '{[':
+            gopts[k] = json.loads(v)        # e.g. group_tiles={"f": [8, 16]}
+        else:
+            gopts[k] = tuple(int(x) for x in v.split(',')) if ',' in v else int(v)
     if args.threads:
         gopts['threads'] = args.threads
     kw = {'literal_mode': args.literal_mode, 'with_data': base is None}
