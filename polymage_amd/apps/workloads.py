@@ -115,10 +115,13 @@ def laplacian(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
     # each thread computing all K for its point (the gray value, its table
     # index and the index arithmetic shared), on 8x16 tiles of their own so
     # K times the LDS still leaves several workgroups per CU
-    # (148.8 -> 141.2 us per call at the learned schedule)
+    # (148.8 -> 141.2 us per call at the learned schedule); with that, the
+    # larger per-thread blocks of a higher fixed cost per sweep iteration
+    # in the block choice (141.2 -> 139.6 us)
     kgroups = dict(('D_gPyramid_L%d' % j if j > 1 else 'gPyramid_L0', [8, 16])
                    for j in range(1, min(J, 4)))
-    gpu_options = dict({'kblock': 1, 'group_tiles': kgroups}, **(gpu_options or {}))
+    gpu_options = dict({'kblock': 1, 'group_tiles': kgroups, 'iter_cost': 10},
+                       **(gpu_options or {}))
     pipe = _pipe(out, pd, {'R': R, 'C': C}, 'laplacian', list(tiles), group_size,
                  gpu_options=gpu_options, literal_mode=literal_mode, inline=funcs)
     vals = {'R': R, 'C': C, 'alpha': 1.0 / (K - 1), 'beta': 1.0}
