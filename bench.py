@@ -68,6 +68,9 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baselines')
     ap.add_argument('--no-size-classes', action='store_true',
                     help='campipe48: skip the size-classed batch (one launch per class)')
+    ap.add_argument('--settle-s', type=float, default=None,
+                    help='untimed launches before each run\'s first timing (default %.2f s)'
+                    % DeviceRun.SETTLE_S)
     ap.add_argument('--cpu-seconds', type=float, default=4.0,
                     help='time budget of each all-core CPU baseline sample')
     ap.add_argument('--traffic', default=os.path.join(ROOT, 'profiles', 'traffic.json'),
@@ -634,6 +637,8 @@ def batch_entry(args, torch, dev, cpu):
 
 def main():
     args = parse()
+    if args.settle_s is not None:
+        DeviceRun.SETTLE_S = args.settle_s
     if args.gpus != 1:
         print('bench.py measures one GPU; --gpus %d is not supported '
               '(no sharding across GPUs)' % args.gpus, file=sys.stderr)
