@@ -84,6 +84,8 @@ def bilateral(rows=2560, cols=1536, tiles=(32, 64), gpu_options=None, with_data=
     from .bilateral import bilateral_grid_full
     pd = {}
     out = bilateral_grid_full(pd)
+    # the slice's blocks without 128-bit LDS row loads (48.1 -> 47.6 us)
+    gpu_options = dict({'no_vlds': 1}, **(gpu_options or {}))
     pipe = _pipe(out, pd, {'rows': rows, 'cols': cols}, 'bilateral', list(tiles), group_size,
                  gpu_options=gpu_options, literal_mode=literal_mode)
     vals = {'rows': rows, 'cols': cols}
